@@ -1,0 +1,1340 @@
+// api.cpp -- the C ABI of include/rbmd_mi355x.h: contexts, systems, device
+// memory, the step schedule and per-kernel timing.  Host code only; kernels
+// are reached through the launch_* functions of dev.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/rbmd_mi355x.h"
+#include "dev.hpp"
+#include "sampler.hpp"
+
+using namespace rbmd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+rbmd_status fail(rbmd_status code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) return fail(RBMD_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+struct Units {
+  double kB, fmt2v, mvv2e, qqr2e, boltz, nktv2p;
+};
+
+// ExecutionMD::InitParameters, ExecutionMD.cpp:329-360
+Units units_of(int u) {
+  Units r;
+  if (u == RBMD_UNIT_REAL) {
+    r.kB = 1.9872067 * std::pow(10.0, -3);
+    r.fmt2v = 4.186 * std::pow(10.0, -4);
+    r.mvv2e = 1.0 / (4.186 * std::pow(10.0, -4));
+    r.qqr2e = 332.06371;
+    r.boltz = 0.0019872067;
+    r.nktv2p = 68568.415;
+  } else if (u == RBMD_UNIT_METAL) {
+    r.kB = 8.617343e-5;
+    r.fmt2v = 1.0 / 1.0364269e-4;
+    r.mvv2e = 1.0364269e-4;
+    r.qqr2e = 14.399645;
+    r.boltz = 8.617343e-5;
+    r.nktv2p = 1.6021765e6;
+  } else {
+    r.kB = r.fmt2v = r.mvv2e = r.qqr2e = r.boltz = r.nktv2p = 1.0;
+  }
+  return r;
+}
+
+struct ProfRec {
+  std::string name;
+  int64_t launches = 0;
+  double ms = 0;
+};
+
+}  // namespace
+
+struct rbmd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool prof = false;
+  std::vector<ProfRec> recs;
+  std::map<std::string, int> index;
+  struct Pending {
+    int rec;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  void flush() {
+    if (pending.empty()) return;
+    (void)hipStreamSynchronize(stream);
+    for (auto& p : pending) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, p.a, p.b);
+      recs[p.rec].ms += ms;
+      recs[p.rec].launches++;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  template <class F>
+  void run(const char* name, F&& f) {
+    if (!prof) {
+      f();
+      return;
+    }
+    auto it = index.find(name);
+    int r;
+    if (it == index.end()) {
+      r = (int)recs.size();
+      index[name] = r;
+      recs.push_back(ProfRec{name, 0, 0});
+    } else {
+      r = it->second;
+    }
+    hipEvent_t a = ev(), b = ev();
+    (void)hipEventRecord(a, stream);
+    f();
+    (void)hipEventRecord(b, stream);
+    pending.push_back({r, a, b});
+    if (pending.size() > 4096) flush();
+  }
+};
+
+struct rbmd_sys {
+  rbmd_ctx* ctx = nullptr;
+  rbmd_params p;
+  Units u;
+  int n = 0, ntypes = 0, nb = 0, na = 0, nmol = 0;
+  SysView v;
+  IntView iv;
+  KView kv;
+  std::vector<void*> allocs;
+  int64_t bytes = 0;
+  // partial-sum rows: pair, kspace, bonded, ke, shake
+  double* rows = nullptr;
+  int nrows = 0, row_pair = 0, row_k = 0, row_bond = 0, row_ke = 0, row_shake = 0;
+  NhcState* nhc = nullptr;
+  NhcParams np;
+  int* shake_iter = nullptr;
+  int* free_dev = nullptr;
+  int nfree = 0;
+  // k space
+  KSampler sampler;
+  int P = 0;
+  double S = 0;
+  double* kcur = nullptr;  // [3P] current k vectors
+  double* kring = nullptr; // [2R][3P]
+  double* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  int ring_R = 256;
+  int64_t ring_batch = -1;  // batch of steps currently staged in half (batch % 2)
+  int64_t ring_have[2] = {-1, -1};
+  bool injected = false;
+  std::vector<double> k_injected;
+  double* rho_part = nullptr;
+  double* rho = nullptr;
+  int rho_part_cap = 0;  // doubles
+  // host mirrors
+  bool locator_built = false, nbr_built = false, initialized = false;
+  double e_long = 0;
+  int64_t energy_step = -1;
+  int64_t step = 0;
+  std::vector<int> host_angles;
+
+  template <class T>
+  rbmd_status dalloc(T** ptr, size_t count) {
+    size_t b = std::max<size_t>(count, 1) * sizeof(T);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, b);
+    if (e != hipSuccess) return fail(RBMD_ERR_OOM, "hipMalloc(%zu bytes): %s", b, hipGetErrorString(e));
+    allocs.push_back(q);
+    bytes += (int64_t)b;
+    *ptr = (T*)q;
+    return RBMD_OK;
+  }
+  template <class T>
+  rbmd_status upload(T** ptr, const T* host, size_t count) {
+    rbmd_status rc = dalloc(ptr, count);
+    if (rc) return rc;
+    if (count && host) HIP_TRY(hipMemcpy(*ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
+    return RBMD_OK;
+  }
+};
+
+namespace {
+
+#define RC_TRY(expr)             \
+  do {                           \
+    rbmd_status _r = (expr);     \
+    if (_r != RBMD_OK) return _r; \
+  } while (0)
+
+rbmd_status check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(RBMD_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  return RBMD_OK;
+}
+
+// ContPointLocator::SetRange/SetCutOff/SetRs (ContPointLocator.h:44-72): nc = ceil(rc/rs),
+// dims = trunc(L / (rc/nc)), dxdydz = L / dims
+void geometry(const rbmd_params& p, const double lo[3], const double hi[3], int dims[3], int* nc, double dx[3]) {
+  double rc = p.cut_off;
+  double rs = p.neighbor_type == RBMD_NEIGHBOR_RBL ? p.r_core : p.cut_off;
+  *nc = (int)std::ceil(rc / rs);
+  for (int d = 0; d < 3; ++d) {
+    dims[d] = (int)((hi[d] - lo[d]) / (rc / *nc));
+    if (dims[d] < 1) dims[d] = 1;
+    dx[d] = (hi[d] - lo[d]) / dims[d];
+  }
+}
+
+// RBL capacities, ExecutionMD.cpp:582-592
+void rbl_capacity(const rbmd_params& p, int n, double volume, int* rs_num, int* pice_num) {
+  double rho = p.rbl_rho > 0 ? p.rbl_rho : n / volume;
+  double rc = p.cut_off, rs = p.r_core;
+  double pif = (double)(float)kPi;
+  double coeff_rcs = 1.0 + (0.05 / rho - 0.05);
+  long id_coeff = std::lround(coeff_rcs);
+  long rsn = (long)(id_coeff * rho * std::ceil(4.0 / 3.0 * pif * rs * rs * rs)) + 1;
+  long rcn = (long)(id_coeff * rho * std::ceil(4.0 / 3.0 * pif * rc * rc * rc)) + 1;
+  double rate = (double)p.neighbor_sample_num / (double)(rcn - rsn);
+  *pice_num = (int)std::ceil(1.0 / rate);
+  // core slot capacity: twice the reference's rs_num plus 32 (the reference
+  // overwrites past rs_num silently, hazard Q7); pice_num keeps its formula
+  *rs_num = (int)(2 * rsn + 32);
+}
+
+int default_cap(const rbmd_sys* s) {
+  double rho = s->n / s->v.volume;
+  double rc = s->p.cut_off;
+  double c = rho * 4.0 / 3.0 * kPi * rc * rc * rc;
+  int cap = (int)(2.0 * c + 64);
+  if (s->v.rbl) cap = s->v.rs_num + (int)(2.0 * c / s->v.pice_num + 64);
+  int st = (2 * s->v.nc + 1);
+  long maxc = (long)std::max(s->n, 1) * st * st * st;
+  return (int)std::min<long>(cap, maxc);
+}
+
+rbmd_status ensure_nbr(rbmd_sys* s, int cap) {
+  if (cap <= 0) cap = default_cap(s);
+  if (s->v.nbr && cap == s->v.cap) return RBMD_OK;
+  if (s->v.nbr) {
+    auto it = std::find(s->allocs.begin(), s->allocs.end(), (void*)s->v.nbr);
+    if (it != s->allocs.end()) s->allocs.erase(it);
+    s->bytes -= (int64_t)s->v.cap * s->n * 4;
+    (void)hipFree(s->v.nbr);
+    s->v.nbr = nullptr;
+  }
+  if (s->v.rbl && cap < s->v.rs_num + 1) return fail(RBMD_ERR_ARG, "RBL capacity %d below rs_num %d", cap, s->v.rs_num);
+  s->v.cap = cap;
+  return s->dalloc(&s->v.nbr, (size_t)cap * s->n);
+}
+
+rbmd_status read_status(rbmd_sys* s, int* st) {
+  HIP_TRY(hipMemcpyAsync(st, s->v.status, sizeof(int), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  return RBMD_OK;
+}
+
+rbmd_status status_to_rc(int st) {
+  if (st & 1) return fail(RBMD_ERR_NONFINITE, "non-finite position, velocity or force");
+  if (st & 2) return fail(RBMD_ERR_SHAKE, "SHAKE/RATTLE did not converge or hit a singular matrix");
+  if (st & 4) return fail(RBMD_ERR_CAPACITY, "a bounded buffer (neighbour list, cell sort, special list) overflowed");
+  return RBMD_OK;
+}
+
+// --------------------------------------------------------------- schedule pieces
+
+void build_cells(rbmd_sys* s, bool bin_from_x) {
+  rbmd_ctx* c = s->ctx;
+  if (bin_from_x) c->run("bin", [&] { launch_bin(s->v, c->stream, s->v.x, s->v.y, s->v.z); });
+  c->run("scan", [&] { launch_scan(s->v, c->stream); });
+  c->run("scatter", [&] { launch_scatter(s->v, c->stream); });
+  c->run("cell_sort", [&] { launch_cell_sort(s->v, c->stream); });
+}
+
+void gather_and_list(rbmd_sys* s) {
+  rbmd_ctx* c = s->ctx;
+  c->run("gather", [&] { launch_gather(s->v, c->stream); });
+  c->run("nbr_build", [&] { launch_nbr_build(s->v, c->stream); });
+  s->locator_built = true;
+  s->nbr_built = true;
+}
+
+bool has_k(const rbmd_sys* s) {
+  return s->p.force_field != RBMD_FF_LJ_CUT && s->p.coulomb_type != RBMD_COULOMB_NONE;
+}
+bool has_bonded(const rbmd_sys* s) { return s->p.force_field == RBMD_FF_CVFF && (s->nb > 0 || s->na > 0); }
+bool shake_on(const rbmd_sys* s) { return s->p.fix_shake == RBMD_SHAKE_TRUE && s->nmol > 0; }
+
+rbmd_status ensure_rho(rbmd_sys* s, int P) {
+  int nblk = (s->n + kBlock * 4 - 1) / (kBlock * 4);
+  size_t need = (size_t)nblk * 2 * P;
+  if ((size_t)s->rho_part_cap < need) {
+    RC_TRY(s->dalloc(&s->rho_part, need));
+    RC_TRY(s->dalloc(&s->rho, 2 * (size_t)P));
+    s->rho_part_cap = (int)need;
+  }
+  s->kv.nblk_rho = nblk;
+  s->kv.rho_part = s->rho_part;
+  s->kv.rho = s->rho;
+  return RBMD_OK;
+}
+
+// Full force evaluation on the current locator (ExecutionNVT::ComputeAllForce,
+// ExecutionNVT.cpp:195-235).  kvec: device pointer to this step's k vectors.
+void compute_force(rbmd_sys* s, const double* kvec, int thermo) {
+  rbmd_ctx* c = s->ctx;
+  double* rows = s->rows;
+  c->run("pair", [&] { launch_pair(s->v, c->stream, thermo, 0, rows + (size_t)s->row_pair * S_NSLOT, nullptr); });
+  if (has_k(s)) {
+    KView k = s->kv;
+    k.P = s->P;
+    k.kvec = kvec;
+    k.S_over_P = s->p.coulomb_type == RBMD_COULOMB_RBE ? s->S / (double)s->P : 1.0;
+    int ewald = s->p.coulomb_type == RBMD_COULOMB_EWALD;
+    c->run("rho_partial", [&] { launch_rho_partial(s->v, k, c->stream); });
+    c->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, c->stream); });
+    c->run("kforce", [&] {
+      launch_kforce(s->v, k, c->stream, ewald, s->p.alpha, thermo, 0, rows + (size_t)s->row_k * S_NSLOT, nullptr);
+    });
+  }
+  if (has_bonded(s))
+    c->run("bonded", [&] { launch_bonded(s->v, s->iv, c->stream, thermo, rows + (size_t)s->row_bond * S_NSLOT, nullptr); });
+}
+
+// k vectors of one MD step (stream id = step index), staged through a
+// double-buffered pinned ring so host sampling overlaps device work
+rbmd_status kvec_for_step(rbmd_sys* s, int64_t step, const double** out) {
+  if (!has_k(s) || s->p.coulomb_type != RBMD_COULOMB_RBE) {
+    *out = s->kcur;
+    return RBMD_OK;
+  }
+  if (s->injected) {
+    *out = s->kcur;
+    return RBMD_OK;
+  }
+  const int R = s->ring_R;
+  const int64_t batch = step / R;
+  const int half = (int)(batch % 2);
+  const size_t per = (size_t)3 * s->P;
+  if (s->ring_have[half] != batch) {
+    HIP_TRY(hipEventSynchronize(s->stage_ev[half]));
+    double* buf = s->stage[half];
+    const int64_t first = batch * R;
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    std::vector<double> mtmp(per * R);
+    for (unsigned t = 0; t < nth; ++t) {
+      th.emplace_back([&, t] {
+        for (int r = (int)t; r < R; r += (int)nth) {
+          s->sampler.draw(s->p.seed, (uint64_t)(first + r), mtmp.data() + r * per);
+          s->sampler.to_k(mtmp.data() + r * per, buf + r * per);
+        }
+      });
+    }
+    for (auto& t : th) t.join();
+    HIP_TRY(hipMemcpyAsync(s->kring + (size_t)half * R * per, buf, per * R * sizeof(double), hipMemcpyHostToDevice,
+                           s->ctx->stream));
+    HIP_TRY(hipEventRecord(s->stage_ev[half], s->ctx->stream));
+    s->ring_have[half] = batch;
+  }
+  *out = s->kring + ((size_t)half * R + (size_t)(step % R)) * per;
+  return RBMD_OK;
+}
+
+void fill_thermo(const rbmd_sys* s, const NhcState& S, int status, int max_iter, rbmd_thermo* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->step = S.step;
+  o->temperature = S.tempT;
+  o->pressure = S.pressure;
+  o->ke_sum = S.tempT_sum;
+  o->e_lj = S.e_lj;
+  o->e_coul = S.e_coul;
+  o->e_long = s->e_long;
+  o->e_bond = S.e_bond;
+  o->e_angle = S.e_angle;
+  for (int k = 0; k < 6; ++k) o->virial[k] = S.virial[k];
+  for (int k = 0; k < 3; ++k) {
+    o->nhc_eta[k] = S.eta[k];
+    o->nhc_eta_dot[k] = S.eta_dot[k];
+  }
+  o->energy_step = s->energy_step;
+  o->shake_max_iter = max_iter;
+  o->status_flags = status;
+}
+
+// e_long = qqr2e (2 pi / V) sum_grid e^{-k^2/4a}|rho|^2/k^2 / N + self / N
+// (ComputeEwaldEnergy + ComputeSelfEnergy, ExecutionMD.cpp:405-443, 508-558)
+rbmd_status long_energy(rbmd_sys* s, const int kmax[3], double* e_avg, double* grid_sum_out) {
+  std::vector<double> kg;
+  for (int i = -kmax[0]; i <= kmax[0]; ++i)
+    for (int j = -kmax[1]; j <= kmax[1]; ++j)
+      for (int k = -kmax[2]; k <= kmax[2]; ++k)
+        if (i || j || k) {
+          kg.push_back(2 * kPi * i / s->v.box[0]);
+          kg.push_back(2 * kPi * j / s->v.box[1]);
+          kg.push_back(2 * kPi * k / s->v.box[2]);
+        }
+  int K = (int)(kg.size() / 3);
+  double* kdev = nullptr;
+  HIP_TRY(hipMalloc(&kdev, kg.size() * sizeof(double)));
+  double *rp = nullptr, *rr = nullptr;
+  int nblk = (s->n + kBlock * 4 - 1) / (kBlock * 4);
+  HIP_TRY(hipMalloc(&rp, (size_t)nblk * 2 * K * sizeof(double)));
+  HIP_TRY(hipMalloc(&rr, (size_t)2 * K * sizeof(double)));
+  HIP_TRY(hipMemcpyAsync(kdev, kg.data(), kg.size() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+  KView k{K, 1.0, kdev, rp, rr, nblk};
+  launch_rho_partial(s->v, k, s->ctx->stream);
+  launch_rho_reduce(s->v, k, s->ctx->stream);
+  std::vector<double> rho((size_t)2 * K);
+  HIP_TRY(hipMemcpyAsync(rho.data(), rr, rho.size() * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+  std::vector<double> q(s->n);
+  HIP_TRY(hipMemcpyAsync(q.data(), s->v.q, s->n * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  (void)hipFree(kdev);
+  (void)hipFree(rp);
+  (void)hipFree(rr);
+  double sum = 0, cmp = 0;
+  for (int l = 0; l < K; ++l) {
+    double kx = kg[3 * l], ky = kg[3 * l + 1], kz = kg[3 * l + 2];
+    double k2 = kx * kx + ky * ky + kz * kz;
+    double term = std::exp(-k2 / (4 * s->p.alpha)) * (rho[2 * l] * rho[2 * l] + rho[2 * l + 1] * rho[2 * l + 1]) / k2;
+    double t = sum + term;
+    cmp += std::fabs(sum) >= std::fabs(term) ? (sum - t) + term : (term - t) + sum;
+    sum = t;
+  }
+  sum += cmp;
+  double q2 = 0;
+  for (int i = 0; i < s->n; ++i) q2 += q[i] * q[i];
+  double self = -std::sqrt(s->p.alpha / kPi) * q2;
+  if (grid_sum_out) *grid_sum_out = sum;
+  *e_avg = s->u.qqr2e * sum * (2 * kPi / s->v.volume) / s->n + s->u.qqr2e * self / s->n;
+  return check_launch("long_energy");
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+const char* rbmd_last_error(void) { return g_last_error.c_str(); }
+const char* rbmd_version(void) { return "rbmd-mi355x 0.1.0 (gfx950)"; }
+
+rbmd_status rbmd_ctx_create(int32_t device, rbmd_ctx** out) {
+  if (!out) return fail(RBMD_ERR_ARG, "null out");
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0) return fail(RBMD_ERR_NODEVICE, "no HIP device visible");
+  if (device < 0 || device >= cnt) return fail(RBMD_ERR_ARG, "device %d out of range (%d devices)", device, cnt);
+  HIP_TRY(hipSetDevice(device));
+  rbmd_ctx* c = new rbmd_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(RBMD_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_ctx_destroy(rbmd_ctx* c) {
+  if (!c) return RBMD_OK;
+  (void)hipSetDevice(c->device);
+  c->flush();
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_ctx_synchronize(rbmd_ctx* c) {
+  if (!c) return fail(RBMD_ERR_ARG, "null ctx");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_footprint_estimate(const rbmd_sys_desc* d, const rbmd_params* p, int64_t* bytes) {
+  if (!d || !p || !bytes || d->n_atoms <= 0) return fail(RBMD_ERR_ARG, "bad arguments");
+  int dims[3], nc;
+  double dx[3];
+  geometry(*p, d->lo, d->hi, dims, &nc, dx);
+  double vol = (d->hi[0] - d->lo[0]) * (d->hi[1] - d->lo[1]) * (d->hi[2] - d->lo[2]);
+  double c = d->n_atoms / vol * 4.0 / 3.0 * kPi * p->cut_off * p->cut_off * p->cut_off;
+  int64_t n = d->n_atoms;
+  int64_t cells = (int64_t)dims[0] * dims[1] * dims[2];
+  int64_t b = n * (8 * 36 + 4 * 12) + cells * 16 + (int64_t)(2.0 * c + 64) * n * 4;
+  b += (int64_t)3 * std::max(p->rbe_p, 1) * 8 * 512 + ((n + 1023) / 1024) * 2 * std::max(p->rbe_p, 1) * 8;
+  *bytes = b;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_params* p, rbmd_sys** out) {
+  if (!ctx || !d || !p || !out) return fail(RBMD_ERR_ARG, "null argument");
+  if (d->n_atoms <= 0 || d->n_types <= 0) return fail(RBMD_ERR_ARG, "n_atoms and n_types must be positive");
+  if (!d->type || !d->charge || !d->mass || !d->eps || !d->sigma) return fail(RBMD_ERR_ARG, "missing per-atom arrays");
+  if (d->n_atoms >= (1 << (31 - kShiftBits))) return fail(RBMD_ERR_UNSUPPORTED, "n_atoms above %d", 1 << (31 - kShiftBits));
+  if (p->cut_off <= 0) return fail(RBMD_ERR_ARG, "cut_off must be positive");
+  if (p->precision != RBMD_PREC_FP64) return fail(RBMD_ERR_UNSUPPORTED, "only RBMD_PREC_FP64 is built");
+  for (int k = 0; k < 3; ++k)
+    if (!(d->hi[k] > d->lo[k])) return fail(RBMD_ERR_ARG, "empty box along axis %d", k);
+  for (int i = 0; i < d->n_atoms; ++i) {
+    if (d->type[i] < 0 || d->type[i] >= d->n_types) return fail(RBMD_ERR_ARG, "atom %d has type %d", i, d->type[i]);
+    if (!(d->mass[i] > 0)) return fail(RBMD_ERR_ARG, "atom %d has non-positive mass", i);
+  }
+  if (p->coulomb_type == RBMD_COULOMB_RBE && p->rbe_p <= 0) return fail(RBMD_ERR_ARG, "coulomb_sample_num must be > 0");
+  if (p->coulomb_type != RBMD_COULOMB_NONE && !(p->alpha > 0)) return fail(RBMD_ERR_ARG, "alpha must be > 0");
+  if (d->special_offsets && d->special_ids && d->special_weights) {
+    if (d->special_offsets[0] != 0) return fail(RBMD_ERR_ARG, "special_offsets[0] must be 0");
+    for (int i = 0; i < d->n_atoms; ++i)
+      if (d->special_offsets[i + 1] < d->special_offsets[i]) return fail(RBMD_ERR_ARG, "special_offsets not monotonic");
+    for (int k = 0; k < d->special_offsets[d->n_atoms]; ++k)
+      if (d->special_ids[k] < 0 || d->special_ids[k] >= d->n_atoms) return fail(RBMD_ERR_ARG, "special id %d out of range", k);
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  rbmd_sys* s = new rbmd_sys();
+  s->ctx = ctx;
+  s->p = *p;
+  s->u = units_of(p->unit);
+  int n = s->n = d->n_atoms;
+  s->ntypes = d->n_types;
+  SysView& v = s->v;
+  std::memset(&v, 0, sizeof(v));
+  v.n = n;
+  v.ntypes = d->n_types;
+  for (int k = 0; k < 3; ++k) {
+    v.lo[k] = d->lo[k];
+    v.hi[k] = d->hi[k];
+    v.box[k] = d->hi[k] - d->lo[k];
+  }
+  v.volume = v.box[0] * v.box[1] * v.box[2];
+  geometry(*p, d->lo, d->hi, v.dims, &v.nc, v.dxdydz);
+  v.ncell = v.dims[0] * v.dims[1] * v.dims[2];
+  v.cut = p->cut_off;
+  v.cut2 = p->cut_off * p->cut_off;
+  v.rs = p->neighbor_type == RBMD_NEIGHBOR_RBL ? p->r_core : p->cut_off;
+  v.rs2 = v.rs * v.rs;
+  v.alpha = p->alpha;
+  v.sqrt_alpha = std::sqrt(p->alpha);
+  v.qqr2e = s->u.qqr2e;
+  v.lj_only = p->force_field == RBMD_FF_LJ_CUT;
+  v.rbl = p->neighbor_type == RBMD_NEIGHBOR_RBL;
+  v.erfc_table = p->erfc_mode == RBMD_ERFC_TABLE;
+  bool have_sp = d->special_offsets && d->special_ids && d->special_weights;
+  v.weighted = (have_sp && p->special_bonds) ? 1 : 0;
+  if (v.rbl) rbl_capacity(*p, n, v.volume, &v.rs_num, &v.pice_num);
+  v.n_items = v.ncell + n / kChunk + 1;
+
+  rbmd_status rc = RBMD_OK;
+  auto A = [&](rbmd_status r) {
+    if (r != RBMD_OK && rc == RBMD_OK) rc = r;
+  };
+  // per-atom state
+  double** fields[] = {&v.x, &v.y, &v.z, &v.vx, &v.vy, &v.vz, &v.fx, &v.fy, &v.fz, &v.ox, &v.oy, &v.oz,
+                       &v.ovx, &v.ovy, &v.ovz, &v.fnx, &v.fny, &v.fnz, &v.fkx, &v.fky, &v.fkz, &v.fbx, &v.fby, &v.fbz};
+  for (double** f : fields) A(s->dalloc(f, n));
+  A(s->upload(&v.q, d->charge, n));
+  A(s->upload(&v.mass, d->mass, n));
+  A(s->upload(&v.type, d->type, n));
+  A(s->dalloc(&v.img, 3 * (size_t)n));
+  // pair tables: eps_ij = sqrt(eps_i eps_j), sigma_ij = (sigma_i + sigma_j)/2 (ExecForceFunction.h:424-431)
+  std::vector<double> eij((size_t)v.ntypes * v.ntypes), s6((size_t)v.ntypes * v.ntypes);
+  for (int a = 0; a < v.ntypes; ++a)
+    for (int b = 0; b < v.ntypes; ++b) {
+      eij[a * v.ntypes + b] = std::sqrt(d->eps[a] * d->eps[b]);
+      double sij = (d->sigma[a] + d->sigma[b]) / 2;
+      s6[a * v.ntypes + b] = sij * sij * sij * sij * sij * sij;
+    }
+  A(s->upload(&v.eps_ij, eij.data(), eij.size()));
+  A(s->upload(&v.sig6_ij, s6.data(), s6.size()));
+  if (have_sp) {
+    int ns = d->special_offsets[n];
+    A(s->upload(&v.sp_off, d->special_offsets, (size_t)n + 1));
+    A(s->upload(&v.sp_ids, d->special_ids, (size_t)std::max(ns, 1)));
+    A(s->upload(&v.sp_w, d->special_weights, (size_t)std::max(ns, 1)));
+  }
+  // cells and lists
+  A(s->dalloc(&v.cell_of, n));
+  A(s->dalloc(&v.cell_count, v.ncell));
+  A(s->dalloc(&v.cell_start, (size_t)v.ncell + 1));
+  A(s->dalloc(&v.cursor, v.ncell));
+  A(s->dalloc(&v.point_ids, n));
+  A(s->dalloc(&v.rank, n));
+  A(s->dalloc(&v.items, v.n_items));
+  A(s->dalloc(&v.n_items_dev, 1));
+  A(s->dalloc(&v.misfit, n));
+  A(s->dalloc(&v.spos, n));
+  A(s->dalloc(&v.stype, n));
+  A(s->dalloc(&v.nbr_num, 2 * (size_t)n));
+  A(s->dalloc(&v.vir_near, 6 * (size_t)n));
+  A(s->dalloc(&v.e_lj, n));
+  A(s->dalloc(&v.e_coul, n));
+  A(s->dalloc(&v.vir_k, 6 * (size_t)n));
+  A(s->dalloc(&v.status, 1));
+  if (rc == RBMD_OK) A(ensure_nbr(s, 0));
+  // bonded terms and SHAKE triplets
+  s->nb = d->n_bonds;
+  s->na = d->n_angles;
+  IntView& iv = s->iv;
+  std::memset(&iv, 0, sizeof(iv));
+  iv.dt = p->dt;
+  iv.fmt2v = s->u.fmt2v;
+  iv.mvv2e = s->u.mvv2e;
+  iv.nb = s->nb;
+  iv.na = s->na;
+  {
+    std::vector<int> boff(n + 1, 0), aoff(n + 1, 0);
+    for (int b = 0; b < s->nb; ++b) {
+      int i0 = d->bond_atoms[2 * b], i1 = d->bond_atoms[2 * b + 1];
+      if (i0 < 0 || i0 >= n || i1 < 0 || i1 >= n) {
+        rc = fail(RBMD_ERR_ARG, "bond %d references atom out of range", b);
+        break;
+      }
+      boff[i0 + 1]++;
+      if (i1 != i0) boff[i1 + 1]++;
+    }
+    for (int a = 0; a < s->na; ++a)
+      for (int t = 0; t < 3; ++t) {
+        int id = d->angle_atoms[3 * a + t];
+        if (id < 0 || id >= n) {
+          rc = fail(RBMD_ERR_ARG, "angle %d references atom out of range", a);
+          break;
+        }
+      }
+    if (rc == RBMD_OK) {
+      std::vector<int> incb, inca;
+      for (int i = 0; i < n; ++i) boff[i + 1] += boff[i];
+      incb.resize(boff[n]);
+      std::vector<int> cur(boff.begin(), boff.end() - 1);
+      for (int b = 0; b < s->nb; ++b) {
+        int i0 = d->bond_atoms[2 * b], i1 = d->bond_atoms[2 * b + 1];
+        incb[cur[i0]++] = b;
+        if (i1 != i0) incb[cur[i1]++] = b;
+      }
+      for (int a = 0; a < s->na; ++a) {
+        int ids[3] = {d->angle_atoms[3 * a], d->angle_atoms[3 * a + 1], d->angle_atoms[3 * a + 2]};
+        for (int t = 0; t < 3; ++t) {
+          bool dup = false;
+          for (int u = 0; u < t; ++u) dup |= ids[u] == ids[t];
+          if (!dup) aoff[ids[t] + 1]++;
+        }
+      }
+      for (int i = 0; i < n; ++i) aoff[i + 1] += aoff[i];
+      inca.resize(aoff[n]);
+      std::vector<int> cura(aoff.begin(), aoff.end() - 1);
+      for (int a = 0; a < s->na; ++a) {
+        int ids[3] = {d->angle_atoms[3 * a], d->angle_atoms[3 * a + 1], d->angle_atoms[3 * a + 2]};
+        for (int t = 0; t < 3; ++t) {
+          bool dup = false;
+          for (int u = 0; u < t; ++u) dup |= ids[u] == ids[t];
+          if (!dup) inca[cura[ids[t]]++] = a;
+        }
+      }
+      int* tmp;
+      A(s->upload(&tmp, boff.data(), boff.size()));
+      iv.inc_b_off = tmp;
+      A(s->upload(&tmp, incb.data(), std::max<size_t>(incb.size(), 1)));
+      iv.inc_b = tmp;
+      A(s->upload(&tmp, aoff.data(), aoff.size()));
+      iv.inc_a_off = tmp;
+      A(s->upload(&tmp, inca.data(), std::max<size_t>(inca.size(), 1)));
+      iv.inc_a = tmp;
+      if (s->nb > 0) {
+        A(s->upload(&tmp, d->bond_atoms, 2 * (size_t)s->nb));
+        iv.bond_atoms = tmp;
+        double* tk;
+        A(s->upload(&tk, d->bond_k, s->nb));
+        iv.bond_k = tk;
+        A(s->upload(&tk, d->bond_r0, s->nb));
+        iv.bond_r0 = tk;
+      }
+      if (s->na > 0) {
+        A(s->upload(&tmp, d->angle_atoms, 3 * (size_t)s->na));
+        iv.angle_all = tmp;
+        double* tk;
+        A(s->upload(&tk, d->angle_k, s->na));
+        iv.angle_k = tk;
+        A(s->upload(&tk, d->angle_theta0, s->na));
+        iv.angle_t0 = tk;
+        s->host_angles.assign(d->angle_atoms, d->angle_atoms + 3 * (size_t)s->na);
+      }
+    }
+  }
+  if (p->fix_shake == RBMD_SHAKE_TRUE && s->na > 0) {
+    s->nmol = s->na;
+    iv.nmol = s->na;
+    iv.angle_atoms = iv.angle_all;
+    std::vector<char> in(n, 0);
+    for (int a = 0; a < s->na; ++a)
+      for (int t = 0; t < 3; ++t) in[d->angle_atoms[3 * a + t]] = 1;
+    std::vector<int> fr;
+    for (int i = 0; i < n; ++i)
+      if (!in[i]) fr.push_back(i);
+    s->nfree = (int)fr.size();
+    if (s->nfree) A(s->upload(&s->free_dev, fr.data(), fr.size()));
+  }
+  // reduction rows
+  int nblk = (n + kBlock - 1) / kBlock;
+  int sblk = (std::max(s->nmol, 1) + kBlock - 1) / kBlock;
+  s->row_pair = 0;
+  s->row_k = nblk;
+  s->row_bond = 2 * nblk;
+  s->row_ke = 3 * nblk;
+  s->row_shake = 4 * nblk;
+  s->nrows = 4 * nblk + sblk;
+  A(s->dalloc(&s->rows, (size_t)s->nrows * S_NSLOT));
+  A(s->dalloc(&s->nhc, 1));
+  A(s->dalloc(&s->shake_iter, 1));
+  // k space
+  if (p->coulomb_type != RBMD_COULOMB_NONE && !v.lj_only) {
+    s->P = p->coulomb_type == RBMD_COULOMB_RBE ? p->rbe_p : 1;
+    if (p->coulomb_type == RBMD_COULOMB_RBE) {
+      s->sampler = KSampler(p->alpha, v.box, p->rbe_p);
+      s->S = s->sampler.S();
+      A(s->dalloc(&s->kcur, 3 * (size_t)s->P));
+      A(s->dalloc(&s->kring, (size_t)2 * s->ring_R * 3 * s->P));
+      for (int h = 0; h < 2; ++h) {
+        hipError_t e = hipHostMalloc((void**)&s->stage[h], (size_t)s->ring_R * 3 * s->P * sizeof(double));
+        if (e != hipSuccess) A(fail(RBMD_ERR_OOM, "hipHostMalloc staging"));
+        e = hipEventCreateWithFlags(&s->stage_ev[h], hipEventDisableTiming);
+        if (e != hipSuccess) A(fail(RBMD_ERR_HIP, "hipEventCreate"));
+      }
+      A(ensure_rho(s, s->P));
+    } else {
+      // Ewald force over the kmax grid in the reference's k order
+      std::vector<double> kg;
+      for (int i = -p->kmax[0]; i <= p->kmax[0]; ++i)
+        for (int j = -p->kmax[1]; j <= p->kmax[1]; ++j)
+          for (int k = -p->kmax[2]; k <= p->kmax[2]; ++k)
+            if (i || j || k) {
+              kg.push_back(2 * kPi * i / v.box[0]);
+              kg.push_back(2 * kPi * j / v.box[1]);
+              kg.push_back(2 * kPi * k / v.box[2]);
+            }
+      s->P = (int)(kg.size() / 3);
+      A(s->upload(&s->kcur, kg.data(), kg.size()));
+      A(ensure_rho(s, s->P));
+    }
+  }
+  if (rc != RBMD_OK) {
+    rbmd_sys_destroy(s);
+    return rc;
+  }
+  hipStream_t st = ctx->stream;
+  launch_fill_int(v.cell_count, 0, v.ncell, st);
+  launch_fill_int(v.status, 0, 1, st);
+  launch_fill_int(v.img, 0, 3 * n, st);
+  launch_fill_double(s->rows, 0.0, s->nrows * S_NSLOT, st);
+  launch_fill_double(v.fkx, 0.0, n, st);
+  launch_fill_double(v.fky, 0.0, n, st);
+  launch_fill_double(v.fkz, 0.0, n, st);
+  launch_fill_double(v.fbx, 0.0, n, st);
+  launch_fill_double(v.fby, 0.0, n, st);
+  launch_fill_double(v.fbz, 0.0, n, st);
+  // thermostat parameters (ExecutionNVT::InitialCondition, ExecutionNVT.cpp:139-187)
+  NhcParams& np = s->np;
+  std::memset(&np, 0, sizeof(np));
+  np.tdof = 3 * n - 3;
+  if (p->fix_shake == RBMD_SHAKE_TRUE) np.tdof -= n;
+  np.mode = p->temp_ctrl;
+  np.inbuild = p->inbuild;
+  np.fix_shake = p->fix_shake;
+  np.n = n;
+  np.nb = s->nb;
+  np.na = s->na;
+  np.boltz = s->u.boltz;
+  np.kB = s->u.kB;
+  np.nktv2p = s->u.nktv2p;
+  np.qqr2e = s->u.qqr2e;
+  np.t_start = p->temperature[0];
+  np.t_stop = p->temperature[1];
+  np.t_period = p->temperature[2];
+  np.t_freq = 1 / np.t_period;
+  np.dt = p->dt;
+  np.dthalf = 0.5 * p->dt;
+  np.dt4 = 0.25 * p->dt;
+  np.dt8 = 0.125 * p->dt;
+  np.tdrag = 1.0 - (p->dt * np.t_freq * 0.0 / 1);
+  np.volume_range = v.volume;
+  np.num_steps = p->num_steps;
+  np.lj_only = v.lj_only;
+  np.cvff = p->force_field == RBMD_FF_CVFF;
+  np.shake_vir = p->fix_shake == RBMD_SHAKE_TRUE && s->na > 0;
+  NhcState zero;
+  std::memset(&zero, 0, sizeof(zero));
+  hipError_t e = hipMemcpy(s->nhc, &zero, sizeof(zero), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    rbmd_sys_destroy(s);
+    return fail(RBMD_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+  }
+  rc = check_launch("sys_create");
+  if (rc != RBMD_OK) {
+    rbmd_sys_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_sys_destroy(rbmd_sys* s) {
+  if (!s) return RBMD_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  for (void* q : s->allocs) (void)hipFree(q);
+  for (int h = 0; h < 2; ++h) {
+    if (s->stage[h]) (void)hipHostFree(s->stage[h]);
+    if (s->stage_ev[h]) (void)hipEventDestroy(s->stage_ev[h]);
+  }
+  delete s;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_sys_footprint(const rbmd_sys* s, int64_t* bytes) {
+  if (!s || !bytes) return fail(RBMD_ERR_ARG, "null argument");
+  *bytes = s->bytes;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_sys_upload_state(rbmd_sys* s, const double* x, const double* v, const int32_t* image) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  int n = s->n;
+  std::vector<double> a(n), b(n), c(n);
+  hipStream_t st = s->ctx->stream;
+  auto put = [&](const double* src, double* dx, double* dy, double* dz) -> rbmd_status {
+    for (int i = 0; i < n; ++i) {
+      a[i] = src[3 * i];
+      b[i] = src[3 * i + 1];
+      c[i] = src[3 * i + 2];
+    }
+    HIP_TRY(hipMemcpyAsync(dx, a.data(), n * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dy, b.data(), n * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dz, c.data(), n * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return RBMD_OK;
+  };
+  if (x) RC_TRY(put(x, s->v.x, s->v.y, s->v.z));
+  if (v) RC_TRY(put(v, s->v.vx, s->v.vy, s->v.vz));
+  if (image) {
+    HIP_TRY(hipMemcpyAsync(s->v.img, image, (size_t)3 * n * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  s->locator_built = s->nbr_built = false;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_sys_download_state(rbmd_sys* s, double* x, double* v, double* f, int32_t* image) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  int n = s->n;
+  hipStream_t st = s->ctx->stream;
+  std::vector<double> a(n), b(n), c(n);
+  auto get = [&](double* dst, const double* dx, const double* dy, const double* dz) -> rbmd_status {
+    HIP_TRY(hipMemcpyAsync(a.data(), dx, n * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(b.data(), dy, n * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(c.data(), dz, n * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) {
+      dst[3 * i] = a[i];
+      dst[3 * i + 1] = b[i];
+      dst[3 * i + 2] = c[i];
+    }
+    return RBMD_OK;
+  };
+  if (x) RC_TRY(get(x, s->v.x, s->v.y, s->v.z));
+  if (v) RC_TRY(get(v, s->v.vx, s->v.vy, s->v.vz));
+  if (f) RC_TRY(get(f, s->v.fx, s->v.fy, s->v.fz));
+  if (image) {
+    HIP_TRY(hipMemcpyAsync(image, s->v.img, (size_t)3 * n * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_locator_geometry(const rbmd_sys* s, int32_t dims[3], int32_t* nc, double dx[3]) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  for (int d = 0; d < 3; ++d) {
+    if (dims) dims[d] = s->v.dims[d];
+    if (dx) dx[d] = s->v.dxdydz[d];
+  }
+  if (nc) *nc = s->v.nc;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_locator_build(rbmd_sys* s) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  build_cells(s, true);
+  s->ctx->run("gather", [&] { launch_gather(s->v, s->ctx->stream); });
+  s->locator_built = true;
+  s->nbr_built = false;
+  return check_launch("locator_build");
+}
+
+rbmd_status rbmd_locator_export(rbmd_sys* s, int32_t* cell_of, int32_t* point_ids, int32_t* lower, int32_t* upper) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (!s->locator_built) return fail(RBMD_ERR_STATE, "locator not built");
+  hipStream_t st = s->ctx->stream;
+  int n = s->n, C = s->v.ncell;
+  std::vector<int> start(C + 1);
+  if (cell_of) HIP_TRY(hipMemcpyAsync(cell_of, s->v.cell_of, n * 4, hipMemcpyDeviceToHost, st));
+  if (point_ids) HIP_TRY(hipMemcpyAsync(point_ids, s->v.point_ids, n * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(start.data(), s->v.cell_start, (C + 1) * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int c = 0; c < C; ++c) {
+    if (lower) lower[c] = start[c];
+    if (upper) upper[c] = start[c + 1];
+  }
+  int stt = 0;
+  RC_TRY(read_status(s, &stt));
+  return status_to_rc(stt);
+}
+
+rbmd_status rbmd_neighbors_build(rbmd_sys* s, int32_t cap) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (!s->locator_built) RC_TRY(rbmd_locator_build(s));
+  RC_TRY(ensure_nbr(s, cap));
+  s->ctx->run("nbr_build", [&] { launch_nbr_build(s->v, s->ctx->stream); });
+  s->nbr_built = true;
+  return check_launch("neighbors_build");
+}
+
+rbmd_status rbmd_neighbors_export(rbmd_sys* s, int32_t cap, int32_t* ids, double* offsets, int32_t* num) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (!s->nbr_built) return fail(RBMD_ERR_STATE, "neighbour lists not built");
+  if (cap != s->v.cap) return fail(RBMD_ERR_ARG, "cap %d differs from the built capacity %d", cap, s->v.cap);
+  int n = s->n;
+  hipStream_t st = s->ctx->stream;
+  std::vector<int> nbr((size_t)cap * n), cnt(2 * (size_t)n), pid(n), rank(n);
+  HIP_TRY(hipMemcpyAsync(nbr.data(), s->v.nbr, nbr.size() * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(cnt.data(), s->v.nbr_num, cnt.size() * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(pid.data(), s->v.point_ids, n * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(rank.data(), s->v.rank, n * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int stt = 0;
+  RC_TRY(read_status(s, &stt));
+  const bool rbl = s->v.rbl;
+  for (int a = 0; a < n; ++a) {
+    int i = rank[a];
+    auto emit = [&](int slot) {
+      int e = nbr[(size_t)slot * n + i];
+      int js = e >> kShiftBits;
+      int code = e & ((1 << kShiftBits) - 1);
+      int sh[3] = {(code & 7) - 3, ((code >> 3) & 7) - 3, ((code >> 6) & 7) - 3};
+      if (ids) ids[(size_t)a * cap + slot] = pid[js];
+      if (offsets)
+        for (int d = 0; d < 3; ++d)
+          offsets[((size_t)a * cap + slot) * 3 + d] = s->v.dxdydz[d] * (double)(sh[d] * s->v.dims[d]);
+    };
+    if (ids)
+      for (int k = 0; k < cap; ++k) ids[(size_t)a * cap + k] = -1;
+    if (offsets)
+      for (int k = 0; k < 3 * cap; ++k) offsets[(size_t)a * cap * 3 + k] = 0.0;
+    if (!rbl) {
+      for (int k = 0; k < cnt[i]; ++k) emit(k);
+      if (num) num[a] = cnt[i];
+    } else {
+      for (int k = 0; k < cnt[2 * i]; ++k) emit(k);
+      for (int k = 0; k < cnt[2 * i + 1]; ++k) emit(s->v.rs_num + k);
+      if (num) {
+        num[2 * a] = cnt[2 * i];
+        num[2 * a + 1] = cnt[2 * i + 1];
+      }
+    }
+  }
+  return status_to_rc(stt);
+}
+
+rbmd_status rbmd_neighbors_capacity(const rbmd_sys* s, int32_t* cap) {
+  if (!s || !cap) return fail(RBMD_ERR_ARG, "bad arguments");
+  *cap = s->v.cap;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_neighbors_count(rbmd_sys* s, int64_t* entries) {
+  if (!s || !entries) return fail(RBMD_ERR_ARG, "bad arguments");
+  if (!s->nbr_built) return fail(RBMD_ERR_STATE, "neighbour lists not built");
+  int m = s->v.rbl ? 2 * s->n : s->n;
+  std::vector<int> cnt(m);
+  HIP_TRY(hipMemcpyAsync(cnt.data(), s->v.nbr_num, (size_t)m * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  int64_t t = 0;
+  for (int c : cnt) t += c;
+  *entries = t;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_force_real(rbmd_sys* s, double* force, double* vir, double* elj, double* ecoul) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (!s->nbr_built) RC_TRY(rbmd_neighbors_build(s, s->v.cap));
+  hipStream_t st = s->ctx->stream;
+  s->ctx->run("pair", [&] { launch_pair(s->v, st, 1, 1, s->rows + (size_t)s->row_pair * S_NSLOT, nullptr); });
+  RC_TRY(check_launch("force_real"));
+  int n = s->n;
+  std::vector<double> fx(n), fy(n), fz(n), vv(6 * (size_t)n), el(n), ec(n);
+  std::vector<int> rank(n);
+  HIP_TRY(hipMemcpyAsync(fx.data(), s->v.fnx, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fy.data(), s->v.fny, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fz.data(), s->v.fnz, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(vv.data(), s->v.vir_near, vv.size() * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(el.data(), s->v.e_lj, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(ec.data(), s->v.e_coul, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(rank.data(), s->v.rank, n * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int a = 0; a < n; ++a) {
+    int i = rank[a];
+    if (force) {
+      force[3 * a] = fx[i];
+      force[3 * a + 1] = fy[i];
+      force[3 * a + 2] = fz[i];
+    }
+    if (vir)
+      for (int k = 0; k < 6; ++k) vir[6 * a + k] = vv[6 * (size_t)i + k];
+    if (elj) elj[a] = s->v.rbl ? 0.0 : el[i];
+    if (ecoul) ecoul[a] = s->v.rbl ? 0.0 : ec[i];
+  }
+  int stt = 0;
+  RC_TRY(read_status(s, &stt));
+  return status_to_rc(stt);
+}
+
+rbmd_status rbmd_rbe_S(double alpha, const double box[3], double* S) {
+  if (!box || !S || !(alpha > 0)) return fail(RBMD_ERR_ARG, "bad arguments");
+  *S = rbmd::rbe_S(alpha, box);
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_rbe_sample_host(double alpha, const double box[3], int32_t p, uint64_t seed, uint64_t stream,
+                                 double* m_out) {
+  if (!box || !m_out || p <= 0 || !(alpha > 0)) return fail(RBMD_ERR_ARG, "bad arguments");
+  KSampler ks(alpha, box, p);
+  ks.draw(seed, stream, m_out);
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_rbe_set_kvectors(rbmd_sys* s, int32_t p, const double* m) {
+  if (!s || !m || p <= 0) return fail(RBMD_ERR_ARG, "bad arguments");
+  if (s->p.coulomb_type != RBMD_COULOMB_RBE) return fail(RBMD_ERR_STATE, "coulomb type is not RBE");
+  if (p != s->P) {
+    s->P = p;
+    RC_TRY(s->dalloc(&s->kcur, 3 * (size_t)p));
+    RC_TRY(ensure_rho(s, p));
+  }
+  std::vector<double> k(3 * (size_t)p);
+  for (int l = 0; l < p; ++l)
+    for (int d = 0; d < 3; ++d) k[3 * l + d] = 2 * kPi * m[3 * l + d] / s->v.box[d];
+  HIP_TRY(hipMemcpy(s->kcur, k.data(), k.size() * 8, hipMemcpyHostToDevice));
+  s->injected = true;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_rbe_sample(rbmd_sys* s, uint64_t seed, uint64_t stream, double* m_out) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (s->p.coulomb_type != RBMD_COULOMB_RBE) return fail(RBMD_ERR_STATE, "coulomb type is not RBE");
+  std::vector<double> m(3 * (size_t)s->sampler.P());
+  s->sampler.draw(seed, stream, m.data());
+  if (m_out) std::copy(m.begin(), m.end(), m_out);
+  return rbmd_rbe_set_kvectors(s, s->sampler.P(), m.data());
+}
+
+rbmd_status rbmd_rbe_structure_factor(rbmd_sys* s, double* rho_out) {
+  if (!s || !rho_out) return fail(RBMD_ERR_ARG, "bad arguments");
+  if (!has_k(s) || !s->kcur) return fail(RBMD_ERR_STATE, "no k vectors");
+  KView k = s->kv;
+  k.P = s->P;
+  k.kvec = s->kcur;
+  hipStream_t st = s->ctx->stream;
+  s->ctx->run("rho_partial", [&] { launch_rho_partial(s->v, k, st); });
+  s->ctx->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, st); });
+  RC_TRY(check_launch("structure_factor"));
+  HIP_TRY(hipMemcpyAsync(rho_out, s->rho, 2 * (size_t)s->P * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return RBMD_OK;
+}
+
+static rbmd_status kforce_out(rbmd_sys* s, int ewald, double* force, double* vir) {
+  KView k = s->kv;
+  k.P = s->P;
+  k.kvec = s->kcur;
+  k.S_over_P = ewald ? 1.0 : s->S / (double)s->P;
+  hipStream_t st = s->ctx->stream;
+  s->ctx->run("rho_partial", [&] { launch_rho_partial(s->v, k, st); });
+  s->ctx->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, st); });
+  s->ctx->run("kforce", [&] {
+    launch_kforce(s->v, k, st, ewald, s->p.alpha, 1, 1, s->rows + (size_t)s->row_k * S_NSLOT, nullptr);
+  });
+  RC_TRY(check_launch("kforce"));
+  int n = s->n;
+  std::vector<double> fx(n), fy(n), fz(n);
+  HIP_TRY(hipMemcpyAsync(fx.data(), s->v.fkx, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fy.data(), s->v.fky, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fz.data(), s->v.fkz, n * 8, hipMemcpyDeviceToHost, st));
+  if (vir) HIP_TRY(hipMemcpyAsync(vir, s->v.vir_k, 6 * (size_t)n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (force)
+    for (int i = 0; i < n; ++i) {
+      force[3 * i] = fx[i];
+      force[3 * i + 1] = fy[i];
+      force[3 * i + 2] = fz[i];
+    }
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_rbe_force(rbmd_sys* s, double* force, double* vir) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (s->p.coulomb_type != RBMD_COULOMB_RBE || !s->kcur) return fail(RBMD_ERR_STATE, "no RBE k vectors");
+  return kforce_out(s, 0, force, vir);
+}
+
+rbmd_status rbmd_ewald_force(rbmd_sys* s, const int32_t kmax[3], double* force, double* vir, double* energy_total) {
+  if (!s || !kmax) return fail(RBMD_ERR_ARG, "bad arguments");
+  if (s->v.lj_only) return fail(RBMD_ERR_STATE, "LJ/CUT has no k space");
+  std::vector<double> kg;
+  for (int i = -kmax[0]; i <= kmax[0]; ++i)
+    for (int j = -kmax[1]; j <= kmax[1]; ++j)
+      for (int k = -kmax[2]; k <= kmax[2]; ++k)
+        if (i || j || k) {
+          kg.push_back(2 * kPi * i / s->v.box[0]);
+          kg.push_back(2 * kPi * j / s->v.box[1]);
+          kg.push_back(2 * kPi * k / s->v.box[2]);
+        }
+  int K = (int)(kg.size() / 3);
+  double* saved_k = s->kcur;
+  int saved_P = s->P;
+  double* kdev = nullptr;
+  HIP_TRY(hipMalloc(&kdev, kg.size() * 8));
+  HIP_TRY(hipMemcpy(kdev, kg.data(), kg.size() * 8, hipMemcpyHostToDevice));
+  s->P = K;
+  s->kcur = kdev;
+  rbmd_status rc = ensure_rho(s, K);
+  if (rc == RBMD_OK) rc = kforce_out(s, 1, force, vir);
+  s->kcur = saved_k;
+  s->P = saved_P;
+  (void)hipFree(kdev);
+  if (rc != RBMD_OK) return rc;
+  if (energy_total) {
+    double e_avg = 0, grid = 0;
+    RC_TRY(long_energy(s, kmax, &e_avg, &grid));
+    *energy_total = s->u.qqr2e * grid * (2 * kPi / s->v.volume);
+  }
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_force_bonded(rbmd_sys* s, double* force, double* eb, double* ea) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  hipStream_t st = s->ctx->stream;
+  int nblk = 0;
+  s->ctx->run("bonded", [&] { launch_bonded(s->v, s->iv, st, 1, s->rows + (size_t)s->row_bond * S_NSLOT, &nblk); });
+  RC_TRY(check_launch("bonded"));
+  int n = s->n;
+  std::vector<double> fx(n), fy(n), fz(n), rows((size_t)nblk * S_NSLOT);
+  HIP_TRY(hipMemcpyAsync(fx.data(), s->v.fbx, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fy.data(), s->v.fby, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fz.data(), s->v.fbz, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(rows.data(), s->rows + (size_t)s->row_bond * S_NSLOT, rows.size() * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (force)
+    for (int i = 0; i < n; ++i) {
+      force[3 * i] = fx[i];
+      force[3 * i + 1] = fy[i];
+      force[3 * i + 2] = fz[i];
+    }
+  double b = 0, a = 0;
+  for (int r = 0; r < nblk; ++r) {
+    b += rows[(size_t)r * S_NSLOT + S_EBOND];
+    a += rows[(size_t)r * S_NSLOT + S_EANGLE];
+  }
+  if (eb) *eb = b;
+  if (ea) *ea = a;
+  return RBMD_OK;
+}
+
+// ExecutionNVT::Init: locator, k-set of step 0, dof/NHC SetUp, first force
+rbmd_status rbmd_md_init(rbmd_sys* s) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  rbmd_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  launch_fill_int(s->v.status, 0, 1, st);
+  s->injected = false;
+  NhcState zero;
+  std::memset(&zero, 0, sizeof(zero));
+  HIP_TRY(hipMemcpyAsync(s->nhc, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+  build_cells(s, true);
+  gather_and_list(s);
+  const double* kv = nullptr;
+  RC_TRY(kvec_for_step(s, 0, &kv));
+  compute_force(s, kv, 1);
+  c->run("second_half", [&] {
+    launch_second_half(s->v, s->iv, st, 0, has_k(s), has_bonded(s), s->rows + (size_t)s->row_ke * S_NSLOT, nullptr);
+  });
+  c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, s->rows, s->nrows, 3, st); });
+  RC_TRY(check_launch("md_init"));
+  s->step = 0;
+  s->initialized = true;
+  if (s->p.energy_every > 0 && has_k(s)) {
+    RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
+    s->energy_step = 0;
+  }
+  int stt = 0;
+  RC_TRY(read_status(s, &stt));
+  return status_to_rc(stt);
+}
+
+static rbmd_status md_one_step(rbmd_sys* s) {
+  rbmd_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  const int64_t step = s->step + 1;
+  const double* kv = nullptr;
+  RC_TRY(kvec_for_step(s, step, &kv));
+  const int tc = s->p.temp_ctrl;
+  const bool nh = tc == RBMD_TEMP_NOSE_HOOVER || tc == RBMD_TEMP_NONE;
+  const bool shake = shake_on(s);
+  double* rows = s->rows;
+  c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows, s->nrows, nh ? 1 : 5, st); });
+  const double* scale = (tc == RBMD_TEMP_NOSE_HOOVER) ? &s->nhc->factor : nullptr;
+  c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, 1); });
+  build_cells(s, false);
+  if (shake) {
+    launch_fill_int(s->shake_iter, 0, 1, st);
+    c->run("shake", [&] { launch_shake(s->v, s->iv, st, rows + (size_t)s->row_shake * S_NSLOT, s->shake_iter); });
+    if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 1);
+  }
+  gather_and_list(s);
+  compute_force(s, kv, 1);
+  c->run("second_half", [&] {
+    launch_second_half(s->v, s->iv, st, 1, has_k(s), has_bonded(s), rows + (size_t)s->row_ke * S_NSLOT, nullptr);
+  });
+  if (nh) {
+    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows, s->nrows, 2, st); });
+    if (tc == RBMD_TEMP_NOSE_HOOVER && !(shake && s->nfree == 0))
+      c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
+    if (shake) {
+      c->run("rattle", [&] { launch_rattle(s->v, s->iv, st); });
+      if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 0);
+    }
+  } else {
+    if (shake) {
+      c->run("rattle", [&] { launch_rattle(s->v, s->iv, st); });
+      if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 0);
+      c->run("ke_rows", [&] { launch_ke_rows(s->v, s->iv, st, rows + (size_t)s->row_ke * S_NSLOT); });
+    }
+    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows, s->nrows, 4, st); });
+    c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
+  }
+  s->step = step;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_md_step(rbmd_sys* s, int64_t n_steps) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (!s->initialized) return fail(RBMD_ERR_STATE, "rbmd_md_init has not run");
+  for (int64_t k = 0; k < n_steps; ++k) {
+    RC_TRY(md_one_step(s));
+    if (s->p.energy_every > 0 && has_k(s) && s->step % s->p.energy_every == 0) {
+      RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
+      s->energy_step = s->step;
+    }
+  }
+  RC_TRY(check_launch("md_step"));
+  int stt = 0;
+  RC_TRY(read_status(s, &stt));
+  return status_to_rc(stt);
+}
+
+rbmd_status rbmd_md_run_graph(rbmd_sys* s, int64_t n_steps) { return rbmd_md_step(s, n_steps); }
+
+rbmd_status rbmd_md_thermo(rbmd_sys* s, rbmd_thermo* o) {
+  if (!s || !o) return fail(RBMD_ERR_ARG, "bad arguments");
+  NhcState S;
+  int stt = 0, it = 0;
+  HIP_TRY(hipMemcpyAsync(&S, s->nhc, sizeof(S), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipMemcpyAsync(&stt, s->v.status, 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipMemcpyAsync(&it, s->shake_iter, 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  fill_thermo(s, S, stt, it, o);
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_md_energy(rbmd_sys* s) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (!has_k(s)) return fail(RBMD_ERR_STATE, "no coulomb k space");
+  RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
+  s->energy_step = s->step;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_profile_enable(rbmd_ctx* c, int32_t on) {
+  if (!c) return fail(RBMD_ERR_ARG, "null ctx");
+  c->flush();
+  c->prof = on != 0;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_profile_reset(rbmd_ctx* c) {
+  if (!c) return fail(RBMD_ERR_ARG, "null ctx");
+  c->flush();
+  for (auto& r : c->recs) {
+    r.launches = 0;
+    r.ms = 0;
+  }
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_profile_get(rbmd_ctx* c, int32_t i, const char** name, int64_t* launches, double* ms) {
+  if (!c) return fail(RBMD_ERR_ARG, "null ctx");
+  c->flush();
+  if (i < 0 || i >= (int)c->recs.size()) return fail(RBMD_ERR_ARG, "profile index %d past the end", i);
+  if (name) *name = c->recs[i].name.c_str();
+  if (launches) *launches = c->recs[i].launches;
+  if (ms) *ms = c->recs[i].ms;
+  return RBMD_OK;
+}
+
+// sizes of the ABI structs, checked from Python against the ctypes mirror
+int32_t rbmd_abi_sizeof(int32_t which) {
+  switch (which) {
+    case 0:
+      return (int32_t)sizeof(rbmd_sys_desc);
+    case 1:
+      return (int32_t)sizeof(rbmd_params);
+    case 2:
+      return (int32_t)sizeof(rbmd_thermo);
+    default:
+      return -1;
+  }
+}
+
+}  // extern "C"

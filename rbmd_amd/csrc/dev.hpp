@@ -1,0 +1,162 @@
+// dev.hpp -- shared device-side types and kernel launcher declarations of the
+// MI355X force path.  Kernels live in k_*.hip; api.cpp orchestrates them.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rbmd {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kBlock = 256;     // threads per workgroup (4 wavefronts of 64)
+constexpr int kWave = 64;       // CDNA wavefront width
+constexpr int kChunk = 64;      // i-atoms per neighbour-build work item (one wavefront)
+constexpr int kMaxSpecial = 8;  // special-list entries held in registers per atom
+constexpr int kShiftBits = 9;   // 3 bits per axis image shift in packed neighbour entries
+
+// Everything a kernel needs to know about one system.  Passed by value.
+struct SysView {
+  int n;            // atoms
+  int ntypes;
+  int ncell;        // dims0*dims1*dims2
+  int dims[3];
+  int nc;           // stencil half width (ceil(rc / rs))
+  int cap;          // neighbour-list capacity per atom
+  int n_items;      // capacity of the work-item list
+  int weighted;     // special-bond weights in the pair term
+  int lj_only;      // LJ/CUT force field
+  int rbl;          // random-batch list
+  int erfc_table;   // step-table erfc (alpha = 0.1)
+  int rs_num, pice_num;
+  double lo[3], hi[3], box[3], dxdydz[3];
+  double volume, cut, cut2, rs, rs2, alpha, sqrt_alpha, qqr2e;
+  // per atom, original order (SoA)
+  double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz;
+  double *ox, *oy, *oz, *ovx, *ovy, *ovz;
+  double *q, *mass;
+  int *type, *img;
+  // per type pair tables [ntypes*ntypes]
+  double *eps_ij, *sig6_ij;
+  // special lists (CSR) and molecule groups
+  int *sp_off, *sp_ids;
+  double *sp_w;
+  // cell list
+  int *cell_of;     // [n] binned cell (clamped)
+  int *cell_count;  // [ncell]
+  int *cell_start;  // [ncell+1]
+  int *cursor;      // [ncell]
+  int *point_ids;   // [n] sorted position -> atom id
+  int *rank;        // [n] atom id -> sorted position
+  int *items;       // [n_items] packed (cell << 8 | chunk)
+  int *n_items_dev; // [1]
+  int *misfit;      // [n] sorted position flag
+  // sorted copies for the pair kernels
+  double4* spos;    // [n] x,y,z,q in sorted order
+  int* stype;       // [n]
+  // neighbour lists, column-major [cap][n] of packed (j_sorted << 9 | shift code)
+  int* nbr;
+  int* nbr_num;     // [n] (VERLET) or [2n] (RBL: core, shell)
+  // per-atom force components (original order)
+  double *fnx, *fny, *fnz;  // real space, sorted order
+  double *fkx, *fky, *fkz;  // k space
+  double *fbx, *fby, *fbz;  // bonded
+  // per-atom optional outputs (sorted order for real space)
+  double* vir_near;  // [6n] or null
+  double* e_lj;      // [n] or null
+  double* e_coul;    // [n] or null
+  double* vir_k;     // [6n] or null
+  // status word: bit0 nonfinite, bit1 shake, bit2 capacity
+  int* status;
+};
+
+// Reduction slots of the scalar block (doubles); see api.cpp for the layout.
+enum ScalarSlot {
+  S_KE = 0,
+  S_ELJ = 1,
+  S_ECOUL = 2,
+  S_VIR_NEAR = 3,   // 6
+  S_VIR_K = 9,      // 6
+  S_VIR_BOND = 15,  // 6
+  S_VIR_ANGLE = 21, // 6
+  S_VIR_SHAKE = 27, // 6
+  S_EBOND = 33,
+  S_EANGLE = 34,
+  S_NSLOT = 40
+};
+
+// ---------------------------------------------------------------- cells
+void launch_bin(const SysView& s, hipStream_t st, const double* x, const double* y, const double* z);
+void launch_scan(const SysView& s, hipStream_t st);
+void launch_scatter(const SysView& s, hipStream_t st);
+void launch_cell_sort(const SysView& s, hipStream_t st);
+void launch_gather(const SysView& s, hipStream_t st);
+void launch_nbr_build(const SysView& s, hipStream_t st);
+void launch_fill_int(int* p, int v, int n, hipStream_t st);
+void launch_fill_double(double* p, double v, int n, hipStream_t st);
+
+// ---------------------------------------------------------------- pair
+// partials: [nblocks][S_NSLOT] block sums (may be null when thermo == 0)
+void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, double* partials, int* nblocks);
+
+// ---------------------------------------------------------------- k space
+struct KView {
+  int P;
+  double S_over_P;      // S / P
+  const double* kvec;   // [3P] k = 2 pi m / L (physical units)
+  double* rho_part;     // [nblk_rho][2P]
+  double* rho;          // [2P]
+  int nblk_rho;
+};
+void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st);
+void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st);
+void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, int thermo,
+                   int peratom, double* partials, int* nblocks);
+
+// ---------------------------------------------------------------- integrate
+struct IntView {
+  double dt, fmt2v, mvv2e;
+  int nmol;                 // SHAKE triplets
+  const int* angle_atoms;   // [3*nmol]
+  // bonded incidence (CSR over atoms)
+  int nb, na;
+  const int* bond_atoms;
+  const double *bond_k, *bond_r0;
+  const int* angle_all;     // [3*na]
+  const double *angle_k, *angle_t0;
+  const int *inc_b_off, *inc_b;   // atom -> bonds
+  const int *inc_a_off, *inc_a;   // atom -> angles
+};
+void launch_first_half(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin);
+void launch_shake(const SysView& s, const IntView& iv, hipStream_t st, double* partials, int* max_iter);
+void launch_rattle(const SysView& s, const IntView& iv, hipStream_t st);
+void launch_bonded(const SysView& s, const IntView& iv, hipStream_t st, int thermo, double* partials, int* nblocks);
+void launch_second_half(const SysView& s, const IntView& iv, hipStream_t st, int kick, int has_k, int has_b,
+                        double* partials, int* nblocks);
+void launch_scale_v(const SysView& s, hipStream_t st, const double* scale);
+void launch_ke_rows(const SysView& s, const IntView& iv, hipStream_t st, double* rows);
+void launch_unconstrained_copy(const SysView& s, hipStream_t st, const int* free_ids, int nfree, int pos);
+
+// NHC / thermostat scalar state on the device (one thread)
+struct NhcState {
+  double eta[3], eta_dot[4], eta_dotdot[3], eta_mass[3];
+  double t_target, ke_target;
+  double tempT, tempT_sum, pressure;
+  double factor;         // last velocity scale
+  double virial[6];
+  double e_lj, e_coul, e_bond, e_angle;
+  int64_t step;
+};
+struct NhcParams {
+  int tdof, mode;        // mode: 0 NHC, 1 Berendsen, 2 rescale, 3 none
+  int inbuild, fix_shake;
+  int n, nb, na;
+  double boltz, kB, nktv2p, qqr2e;
+  double t_start, t_stop, t_freq, t_period, dt, dthalf, dt4, dt8, tdrag;
+  double volume_range;
+  int64_t num_steps;
+  int lj_only, cvff, shake_vir;
+};
+// phase 1: step++ and NHC half step (InitialIntegrate); 2: ComputeTempe + pressure + NHC
+// (FinalIntegrate); 3: SetUp; 4: Berendsen/rescale end of step; 5: step++ only
+void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const double* rows, int nrows, int phase, hipStream_t st);
+
+}  // namespace rbmd

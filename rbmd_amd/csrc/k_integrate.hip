@@ -1,0 +1,641 @@
+// k_integrate.hip -- velocity Verlet, PBC, SHAKE/RATTLE, bonded terms, force
+// assembly, reductions and the Nose-Hoover chain, all FP64 and device resident.
+//
+//   UpdateVelocity        RunWorklet.cpp:2357-2375 (v += 0.5 F/m dt fmt2v)
+//   UpdatePosition + PBC  RunWorklet.cpp:2446-2490, 2602-2636 (wrap on x < lo / x >= hi)
+//   SHAKE                 MolecularWorklet.h:2367-2645 (NewConstraintAWaterBondAngleWorklet)
+//   RATTLE                MolecularWorklet.h:2647-2832 (NewConstraintBWaterBondAngleWorklet)
+//   bonds / angles        MolecularWorklet.h:58-397 (gathered per atom instead of keyed reduce)
+//   ComputeTempe          ExecutionNVT.cpp:736-773
+//   NHCTempIntegrate      ExecutionNVT.cpp:1172-1256 (run on the device by one thread)
+//   Compute_Pressure      ExecutionNVT.cpp:1258-1276, ComputeVirial ExecutionMD.cpp:1263-1366
+#include "dev.hpp"
+#include "reduce.hpp"
+
+namespace rbmd {
+
+__device__ __forceinline__ int bin_axis_i(double x, double lo, double d, int dims) {
+  double t = (x - lo) / d;
+  int k = 0;
+  if (t == t) k = (int)fmax(fmin(t, 2.0e9), -2.0e9);
+  k = k < 0 ? 0 : k;
+  k = k > dims - 1 ? dims - 1 : k;
+  return k;
+}
+
+// NHC scale (optional), kick, drift, PBC, binning
+__global__ __launch_bounds__(kBlock) void k_first_half(SysView s, IntView iv, const double* scale, int do_bin) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.n) return;
+  double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+  if (scale) {
+    double f = *scale;
+    vx = f * vx;
+    vy = f * vy;
+    vz = f * vz;
+  }
+  s.ovx[i] = vx;
+  s.ovy[i] = vy;
+  s.ovz[i] = vz;
+  double m = s.mass[i];
+  vx += 0.5 * s.fx[i] / m * iv.dt * iv.fmt2v;
+  vy += 0.5 * s.fy[i] / m * iv.dt * iv.fmt2v;
+  vz += 0.5 * s.fz[i] / m * iv.dt * iv.fmt2v;
+  s.vx[i] = vx;
+  s.vy[i] = vy;
+  s.vz[i] = vz;
+  double p[3] = {s.x[i], s.y[i], s.z[i]};
+  s.ox[i] = p[0];
+  s.oy[i] = p[1];
+  s.oz[i] = p[2];
+  p[0] += vx * iv.dt;
+  p[1] += vy * iv.dt;
+  p[2] += vz * iv.dt;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (p[d] < s.lo[d]) {
+      p[d] += s.box[d];
+      s.img[3 * i + d] -= 1;
+    } else if (p[d] >= s.hi[d]) {
+      p[d] -= s.box[d];
+      s.img[3 * i + d] += 1;
+    }
+  }
+  s.x[i] = p[0];
+  s.y[i] = p[1];
+  s.z[i] = p[2];
+  if (!isfinite(p[0] + p[1] + p[2] + vx + vy + vz)) atomicOr(s.status, 1);
+  if (do_bin) {
+    int a = bin_axis_i(p[0], s.lo[0], s.dxdydz[0], s.dims[0]);
+    int b = bin_axis_i(p[1], s.lo[1], s.dxdydz[1], s.dims[1]);
+    int c = bin_axis_i(p[2], s.lo[2], s.dxdydz[2], s.dims[2]);
+    int cell = a + b * s.dims[0] + c * s.dims[0] * s.dims[1];
+    s.cell_of[i] = cell;
+    atomicAdd(&s.cell_count[cell], 1);
+  }
+}
+
+void launch_first_half(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin) {
+  hipLaunchKernelGGL(k_first_half, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, iv, scale, do_bin);
+}
+
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 add(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 mul(double k, D3 a) { return {k * a.x, k * a.y, k * a.z}; }
+__device__ __forceinline__ D3 divs(D3 a, double k) { return {a.x / k, a.y / k, a.z / k}; }
+__device__ __forceinline__ double dotp(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 mimg(const SysView& s, D3 a, D3 b) {
+  D3 v = sub(a, b);
+  if (fabs(v.x) > s.box[0] * 0.5) v.x -= (v.x > 0 ? s.box[0] : -s.box[0]);
+  if (fabs(v.y) > s.box[1] * 0.5) v.y -= (v.y > 0 ? s.box[1] : -s.box[1]);
+  if (fabs(v.z) > s.box[2] * 0.5) v.z -= (v.z > 0 ? s.box[2] : -s.box[2]);
+  return v;
+}
+__device__ __forceinline__ D3 ld(const double* x, const double* y, const double* z, int i) { return {x[i], y[i], z[i]}; }
+__device__ __forceinline__ void st3(double* x, double* y, double* z, int i, D3 v) {
+  x[i] = v.x;
+  y[i] = v.y;
+  z[i] = v.z;
+}
+
+// SHAKE on (old x, old v, F) of one H-O-H triplet; writes x, v, image flags
+__global__ __launch_bounds__(kBlock) void k_shake(SysView s, IntView iv, double* rows, int* max_iter_out) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  int my_iter = 0;
+  if (a < iv.nmol) {
+    const double dt = iv.dt, fmt2v = iv.fmt2v;
+    int ids[3] = {iv.angle_atoms[3 * a], iv.angle_atoms[3 * a + 1], iv.angle_atoms[3 * a + 2]};
+    D3 ox[3], sp[3], ovv[3], ff[3];
+    double m[3];
+    for (int t = 0; t < 3; ++t) {
+      int id = ids[t];
+      ox[t] = ld(s.ox, s.oy, s.oz, id);
+      ovv[t] = ld(s.ovx, s.ovy, s.ovz, id);
+      ff[t] = ld(s.fx, s.fy, s.fz, id);
+      m[t] = s.mass[id];
+      sp[t] = add(add(ox[t], mul(dt, ovv[t])), mul(fmt2v, divs(mul(0.5 * dt * dt, ff[t]), m[t])));
+    }
+    const double bond1 = 1.0, bond2 = 1.0;
+    const double bond12 = sqrt(bond1 * bond1 + bond2 * bond2 - 2.0 * bond1 * bond2 * cos((109.4700 / 180.0) * kPi));
+    D3 r01 = mimg(s, ox[1], ox[0]);
+    D3 r12 = mimg(s, ox[2], ox[1]);
+    D3 r20 = mimg(s, ox[0], ox[2]);
+    D3 s10 = mimg(s, sp[0], sp[1]);
+    D3 s21 = mimg(s, sp[1], sp[2]);
+    D3 s02 = mimg(s, sp[2], sp[0]);
+    double r01sq = dotp(r01, r01), r02sq = dotp(r20, r20), r12sq = dotp(r12, r12);
+    double s01sq = dotp(s10, s10), s02sq = dotp(s02, s02), s12sq = dotp(s21, s21);
+    double im0 = 1.0 / m[0], im1 = 1.0 / m[1], im2 = 1.0 / m[2];
+    double a11 = 2.0 * (im0 + im1) * dotp(s10, r01);
+    double a12 = -2.0 * im1 * dotp(s10, r12);
+    double a13 = -2.0 * im0 * dotp(s10, r20);
+    double a21 = -2.0 * im1 * dotp(s21, r01);
+    double a22 = 2.0 * (im1 + im2) * dotp(s21, r12);
+    double a23 = -2.0 * im2 * dotp(s21, r20);
+    double a31 = -2.0 * im0 * dotp(s02, r01);
+    double a32 = -2.0 * im2 * dotp(s02, r12);
+    double a33 = 2.0 * (im0 + im2) * dotp(s02, r20);
+    double det = a11 * a22 * a33 + a12 * a23 * a31 + a13 * a21 * a32 - a11 * a23 * a32 - a12 * a21 * a33 - a13 * a22 * a31;
+    bool bad = fabs(det) < 0.0001;
+    double di = 1.0 / det;
+    double b11 = di * (a22 * a33 - a23 * a32), b12 = -di * (a12 * a33 - a13 * a32), b13 = di * (a12 * a23 - a13 * a22);
+    double b21 = -di * (a21 * a33 - a23 * a31), b22 = di * (a11 * a33 - a13 * a31), b23 = -di * (a11 * a23 - a13 * a21);
+    double b31 = di * (a21 * a32 - a22 * a31), b32 = -di * (a11 * a32 - a12 * a31), b33 = di * (a11 * a22 - a12 * a21);
+    double r0120 = dotp(r01, r20), r0112 = dotp(r01, r12), r2012 = dotp(r20, r12);
+    double q1_0101 = (im0 + im1) * (im0 + im1) * r01sq, q1_1212 = im1 * im1 * r12sq, q1_2020 = im0 * im0 * r02sq;
+    double q1_0120 = -2.0 * (im0 + im1) * im0 * r0120, q1_0112 = -2.0 * (im0 + im1) * im1 * r0112;
+    double q1_2012 = 2.0 * im0 * im1 * r2012;
+    double q2_0101 = im1 * im1 * r01sq, q2_1212 = (im1 + im2) * (im1 + im2) * r12sq, q2_2020 = im2 * im2 * r02sq;
+    double q2_0120 = 2.0 * im1 * im2 * r0120, q2_0112 = -2.0 * (im1 + im2) * im1 * r0112;
+    double q2_2012 = -2.0 * (im1 + im2) * im2 * r2012;
+    double q3_0101 = im0 * im0 * r01sq, q3_1212 = im2 * im2 * r12sq, q3_2020 = (im0 + im2) * (im0 + im2) * r02sq;
+    double q3_0120 = -2.0 * (im0 + im2) * im0 * r0120, q3_0112 = 2.0 * im0 * im2 * r0112;
+    double q3_2012 = -2.0 * (im0 + im2) * im2 * r2012;
+    const double tol = 0.00001;
+    const int max_iter = 5000;
+    double l01 = 0, l20 = 0, l12 = 0;
+    int niter = 0, done = 0, ovf = 0;
+    while (!done && niter < max_iter) {
+      double quad1 = q1_0101 * l01 * l01 + q1_2020 * l20 * l20 + q1_1212 * l12 * l12 + q1_0120 * l01 * l20 +
+                     q1_0112 * l01 * l12 + q1_2012 * l20 * l12;
+      double quad2 = q2_0101 * l01 * l01 + q2_2020 * l20 * l20 + q2_1212 * l12 * l12 + q2_0120 * l01 * l20 +
+                     q2_0112 * l01 * l12 + q2_2012 * l20 * l12;
+      double quad3 = q3_0101 * l01 * l01 + q3_2020 * l20 * l20 + q3_1212 * l12 * l12 + q3_0120 * l01 * l20 +
+                     q3_0112 * l01 * l12 + q3_2012 * l20 * l12;
+      double b1 = bond1 * bond1 - s01sq - quad1;
+      double b2 = bond2 * bond2 - s12sq - quad2;
+      double b3 = bond12 * bond12 - s02sq - quad3;
+      double n01 = b11 * b1 + b12 * b2 + b13 * b3;
+      double n12 = b21 * b1 + b22 * b2 + b23 * b3;
+      double n20 = b31 * b1 + b32 * b2 + b33 * b3;
+      done = 1;
+      if (fabs(n01 - l01) > tol) done = 0;
+      if (fabs(n20 - l20) > tol) done = 0;
+      if (fabs(n12 - l12) > tol) done = 0;
+      l01 = n01;
+      l20 = n20;
+      l12 = n12;
+      if (isnan(l01) || isnan(l20) || isnan(l12) || fabs(l01) > 1e20 || fabs(l20) > 1e20 || fabs(l12) > 1e20) {
+        done = 1;
+        ovf = 1;
+      }
+      niter++;
+    }
+    if (!done || ovf || bad) atomicOr(s.status, 2);
+    my_iter = niter;
+    D3 pc[3];
+    pc[0] = sub(mul(im0, mul(l01, r01)), mul(im0, mul(l20, r20)));
+    pc[1] = sub(mul(im1, mul(l12, r12)), mul(im1, mul(l01, r01)));
+    pc[2] = sub(mul(im2, mul(l20, r20)), mul(im2, mul(l12, r12)));
+    for (int t = 0; t < 3; ++t) {
+      int id = ids[t];
+      D3 sv = add(ovv[t], mul(fmt2v, divs(mul(0.5 * dt, ff[t]), m[t])));
+      st3(s.vx, s.vy, s.vz, id, add(sv, divs(pc[t], dt)));
+      D3 np = add(sp[t], pc[t]);
+      double p[3] = {np.x, np.y, np.z};
+      for (int d = 0; d < 3; ++d) {
+        double rng = s.hi[d] - s.lo[d];
+        if (p[d] < s.lo[d]) {
+          p[d] += rng;
+          s.img[3 * id + d] -= 1;
+        }
+        if (p[d] > s.hi[d]) {
+          p[d] -= rng;
+          s.img[3 * id + d] += 1;
+        }
+      }
+      s.x[id] = p[0];
+      s.y[id] = p[1];
+      s.z[id] = p[2];
+    }
+    double dtfsq = 0.5 * dt * dt * fmt2v;
+    l01 /= dtfsq;
+    l20 /= dtfsq;
+    l12 /= dtfsq;
+    vir[0] = l01 * r01.x * r01.x + l20 * r20.x * r20.x + l12 * r12.x * r12.x;
+    vir[1] = l01 * r01.y * r01.y + l20 * r20.y * r20.y + l12 * r12.y * r12.y;
+    vir[2] = l01 * r01.z * r01.z + l20 * r20.z * r20.z + l12 * r12.z * r12.z;
+    vir[3] = l01 * r01.x * r01.y + l20 * r20.x * r20.y + l12 * r12.x * r12.y;
+    vir[4] = l01 * r01.x * r01.z + l20 * r20.x * r20.z + l12 * r12.x * r12.z;
+    vir[5] = l01 * r01.y * r01.z + l20 * r20.y * r20.z + l12 * r12.y * r12.z;
+  }
+  if (max_iter_out) atomicMax(max_iter_out, my_iter);
+  const double vals[6] = {vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
+  const int slots[6] = {S_VIR_SHAKE, S_VIR_SHAKE + 1, S_VIR_SHAKE + 2, S_VIR_SHAKE + 3, S_VIR_SHAKE + 4,
+                        S_VIR_SHAKE + 5};
+  block_write_row<6>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+}
+
+// atoms outside every SHAKE triplet get x = x_old, v = v_old: the reference
+// copies the whole old arrays back after the constraint worklets
+// (ExecutionNVT.cpp:869-870, 896)
+__global__ __launch_bounds__(kBlock) void k_unconstrained_copy(SysView s, const int* free_ids, int nfree, int pos) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nfree) return;
+  int i = free_ids[k];
+  if (pos) {
+    s.x[i] = s.ox[i];
+    s.y[i] = s.oy[i];
+    s.z[i] = s.oz[i];
+  }
+  s.vx[i] = s.ovx[i];
+  s.vy[i] = s.ovy[i];
+  s.vz[i] = s.ovz[i];
+}
+
+void launch_shake(const SysView& s, const IntView& iv, hipStream_t st, double* rows, int* max_iter) {
+  int grid = (iv.nmol + kBlock - 1) / kBlock;
+  if (grid > 0) hipLaunchKernelGGL(k_shake, dim3(grid), dim3(kBlock), 0, st, s, iv, rows, max_iter);
+}
+
+void launch_unconstrained_copy(const SysView& s, hipStream_t st, const int* free_ids, int nfree, int pos) {
+  if (nfree <= 0) return;
+  hipLaunchKernelGGL(k_unconstrained_copy, dim3((nfree + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, free_ids,
+                     nfree, pos);
+}
+
+// RATTLE on (x, old v, F)
+__global__ __launch_bounds__(kBlock) void k_rattle(SysView s, IntView iv) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= iv.nmol) return;
+  const double dt = iv.dt, fmt2v = iv.fmt2v;
+  int i0 = iv.angle_atoms[3 * a], i1 = iv.angle_atoms[3 * a + 1], i2 = iv.angle_atoms[3 * a + 2];
+  int ids[3] = {i0, i1, i2};
+  D3 sv[3];
+  double m[3];
+  for (int t = 0; t < 3; ++t) {
+    int id = ids[t];
+    m[t] = s.mass[id];
+    sv[t] = add(ld(s.ovx, s.ovy, s.ovz, id), mul(fmt2v, divs(mul(0.5 * dt, ld(s.fx, s.fy, s.fz, id)), m[t])));
+  }
+  D3 p0 = ld(s.x, s.y, s.z, i0), p1 = ld(s.x, s.y, s.z, i1), p2 = ld(s.x, s.y, s.z, i2);
+  D3 r01 = mimg(s, p1, p0), r12 = mimg(s, p2, p1), r20 = mimg(s, p0, p2);
+  D3 sv10 = sub(sv[0], sv[1]), sv21 = sub(sv[1], sv[2]), sv02 = sub(sv[2], sv[0]);
+  double im0 = 1.0 / m[0], im1 = 1.0 / m[1], im2 = 1.0 / m[2];
+  double A[3][3];
+  A[0][0] = (im1 + im0) * dotp(r01, r01);
+  A[0][1] = -im1 * dotp(r01, r12);
+  A[0][2] = (-im0) * dotp(r01, r20);
+  A[1][0] = A[0][1];
+  A[1][1] = (im1 + im2) * dotp(r12, r12);
+  A[1][2] = -(im2)*dotp(r20, r12);
+  A[2][0] = A[0][2];
+  A[2][1] = A[1][2];
+  A[2][2] = (im0 + im2) * dotp(r20, r20);
+  double c[3] = {-dotp(sv10, r01), -dotp(sv21, r12), -dotp(sv02, r20)};
+  double det = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1] -
+               A[0][0] * A[1][2] * A[2][1] - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
+  if (fabs(det) < 0.0001) atomicOr(s.status, 2);
+  double di = 1.0 / det;
+  double B[3][3];
+  B[0][0] = di * (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
+  B[0][1] = -di * (A[0][1] * A[2][2] - A[0][2] * A[2][1]);
+  B[0][2] = di * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+  B[1][0] = -di * (A[1][0] * A[2][2] - A[1][2] * A[2][0]);
+  B[1][1] = di * (A[0][0] * A[2][2] - A[0][2] * A[2][0]);
+  B[1][2] = -di * (A[0][0] * A[1][2] - A[0][2] * A[1][0]);
+  B[2][0] = di * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  B[2][1] = -di * (A[0][0] * A[2][1] - A[0][1] * A[2][0]);
+  B[2][2] = di * (A[0][0] * A[1][1] - A[0][1] * A[1][0]);
+  double l[3];
+  for (int r = 0; r < 3; ++r) {
+    l[r] = 0;
+    for (int k = 0; k < 3; ++k) l[r] += B[r][k] * c[k];
+  }
+  D3 vc0 = sub(mul(im0, mul(l[0], r01)), mul(im0, mul(l[2], r20)));
+  D3 vc1 = sub(mul(im1, mul(l[1], r12)), mul(im1, mul(l[0], r01)));
+  D3 vc2 = sub(mul(im2, mul(l[2], r20)), mul(im2, mul(l[1], r12)));
+  st3(s.vx, s.vy, s.vz, i0, add(sv[0], vc0));
+  st3(s.vx, s.vy, s.vz, i1, add(sv[1], vc1));
+  st3(s.vx, s.vy, s.vz, i2, add(sv[2], vc2));
+}
+
+void launch_rattle(const SysView& s, const IntView& iv, hipStream_t st) {
+  int grid = (iv.nmol + kBlock - 1) / kBlock;
+  if (grid > 0) hipLaunchKernelGGL(k_rattle, dim3(grid), dim3(kBlock), 0, st, s, iv);
+}
+
+// bond + angle forces gathered per atom over its incident terms; the energy
+// and virial of a term are counted by its first atom
+template <bool THERMO>
+__global__ __launch_bounds__(kBlock) void k_bonded(SysView s, IntView iv, double* rows) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double f[3] = {0, 0, 0};
+  double eb = 0, ea = 0;
+  double vb[6] = {0, 0, 0, 0, 0, 0}, va[6] = {0, 0, 0, 0, 0, 0};
+  if (i < s.n) {
+    for (int k = iv.inc_b_off[i]; k < iv.inc_b_off[i + 1]; ++k) {
+      int b = iv.inc_b[k];
+      int a0 = iv.bond_atoms[2 * b], a1 = iv.bond_atoms[2 * b + 1];
+      D3 r = mimg(s, ld(s.x, s.y, s.z, a0), ld(s.x, s.y, s.z, a1));
+      double d = sqrt(dotp(r, r));
+      double dr = d - iv.bond_r0[b];
+      double rk = iv.bond_k[b] * dr;
+      double fb = d > 0.01 ? -2.0 * rk / d : 0.0;
+      D3 fi = mul(fb, r);
+      double sg = (a0 == i) ? 1.0 : -1.0;
+      f[0] += sg * fi.x;
+      f[1] += sg * fi.y;
+      f[2] += sg * fi.z;
+      if (THERMO && a0 == i) {
+        eb += rk * dr;
+        vb[0] += r.x * fi.x;
+        vb[1] += r.y * fi.y;
+        vb[2] += r.z * fi.z;
+        vb[3] += r.x * fi.y;
+        vb[4] += r.x * fi.z;
+        vb[5] += r.y * fi.z;
+      }
+    }
+    for (int k = iv.inc_a_off[i]; k < iv.inc_a_off[i + 1]; ++k) {
+      int a = iv.inc_a[k];
+      int ai = iv.angle_all[3 * a], aj = iv.angle_all[3 * a + 1], ak = iv.angle_all[3 * a + 2];
+      D3 pj = ld(s.x, s.y, s.z, aj);
+      D3 rij = mimg(s, ld(s.x, s.y, s.z, ai), pj);
+      D3 rkj = mimg(s, ld(s.x, s.y, s.z, ak), pj);
+      double dij2 = dotp(rij, rij), dij = sqrt(dij2);
+      double dkj2 = dotp(rkj, rkj), dkj = sqrt(dkj2);
+      double c = dotp(rij, rkj);
+      c /= dij * dkj;
+      if (c > 1.0) c = 1.0;
+      if (c < -1.0) c = -1.0;
+      double sn = sqrt(1.0 - c * c);
+      if (sn < 0.001) sn = 0.001;
+      sn = 1.0 / sn;
+      double dth = acos(c) - (iv.angle_t0[a] * kPi) / 180;
+      double tk = iv.angle_k[a] * dth;
+      double aa = -2.0 * tk * sn;
+      double a11 = aa * c / dij2, a12 = -aa / (dij * dkj), a22 = aa * c / dkj2;
+      D3 fi = add(mul(a11, rij), mul(a12, rkj));
+      D3 fk = add(mul(a22, rkj), mul(a12, rij));
+      D3 mine = (i == ai) ? fi : ((i == ak) ? fk : mul(-1.0, add(fi, fk)));
+      f[0] += mine.x;
+      f[1] += mine.y;
+      f[2] += mine.z;
+      if (THERMO && i == ai) {
+        ea += tk * dth;
+        va[0] += rij.x * fi.x + rkj.x * fk.x;
+        va[1] += rij.y * fi.y + rkj.y * fk.y;
+        va[2] += rij.z * fi.z + rkj.z * fk.z;
+        va[3] += rij.x * fi.y + rkj.x * fk.y;
+        va[4] += rij.x * fi.z + rkj.x * fk.z;
+        va[5] += rij.y * fi.z + rkj.y * fk.z;
+      }
+    }
+    s.fbx[i] = f[0];
+    s.fby[i] = f[1];
+    s.fbz[i] = f[2];
+  }
+  if (THERMO) {
+    const double vals[14] = {eb, ea, vb[0], vb[1], vb[2], vb[3], vb[4], vb[5], va[0], va[1], va[2], va[3], va[4], va[5]};
+    const int slots[14] = {S_EBOND,        S_EANGLE,        S_VIR_BOND,      S_VIR_BOND + 1,  S_VIR_BOND + 2,
+                           S_VIR_BOND + 3, S_VIR_BOND + 4,  S_VIR_BOND + 5,  S_VIR_ANGLE,     S_VIR_ANGLE + 1,
+                           S_VIR_ANGLE + 2, S_VIR_ANGLE + 3, S_VIR_ANGLE + 4, S_VIR_ANGLE + 5};
+    block_write_row<14>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+  }
+}
+
+void launch_bonded(const SysView& s, const IntView& iv, hipStream_t st, int thermo, double* rows, int* nblocks) {
+  int grid = (s.n + kBlock - 1) / kBlock;
+  if (nblocks) *nblocks = grid;
+  if (thermo)
+    hipLaunchKernelGGL(k_bonded<true>, dim3(grid), dim3(kBlock), 0, st, s, iv, rows);
+  else
+    hipLaunchKernelGGL(k_bonded<false>, dim3(grid), dim3(kBlock), 0, st, s, iv, rows);
+}
+
+// F = F_real[rank] + F_k + F_bonded; optional second kick; kinetic energy row
+__global__ __launch_bounds__(kBlock) void k_second_half(SysView s, IntView iv, int kick, int has_k, int has_b,
+                                                        double* rows) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0;
+  if (i < s.n) {
+    int r = s.rank[i];
+    double fx = s.fnx[r], fy = s.fny[r], fz = s.fnz[r];
+    if (has_k) {
+      fx = s.fkx[i] + fx;
+      fy = s.fky[i] + fy;
+      fz = s.fkz[i] + fz;
+    }
+    if (has_b) {
+      fx += s.fbx[i];
+      fy += s.fby[i];
+      fz += s.fbz[i];
+    }
+    s.fx[i] = fx;
+    s.fy[i] = fy;
+    s.fz[i] = fz;
+    double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+    double m = s.mass[i];
+    if (kick) {
+      s.ovx[i] = vx;
+      s.ovy[i] = vy;
+      s.ovz[i] = vz;
+      vx += 0.5 * fx / m * iv.dt * iv.fmt2v;
+      vy += 0.5 * fy / m * iv.dt * iv.fmt2v;
+      vz += 0.5 * fz / m * iv.dt * iv.fmt2v;
+      s.vx[i] = vx;
+      s.vy[i] = vy;
+      s.vz[i] = vz;
+    }
+    ke = m * (vx * vx + vy * vy + vz * vz) * iv.mvv2e;
+    if (!isfinite(fx + fy + fz)) atomicOr(s.status, 1);
+  }
+  const double vals[1] = {ke};
+  const int slots[1] = {S_KE};
+  block_write_row<1>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+}
+
+void launch_second_half(const SysView& s, const IntView& iv, hipStream_t st, int kick, int has_k, int has_b,
+                        double* rows, int* nblocks) {
+  int grid = (s.n + kBlock - 1) / kBlock;
+  if (nblocks) *nblocks = grid;
+  hipLaunchKernelGGL(k_second_half, dim3(grid), dim3(kBlock), 0, st, s, iv, kick, has_k, has_b, rows);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ke_rows(SysView s, IntView iv, double* rows) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0;
+  if (i < s.n) ke = s.mass[i] * (s.vx[i] * s.vx[i] + s.vy[i] * s.vy[i] + s.vz[i] * s.vz[i]) * iv.mvv2e;
+  const double vals[1] = {ke};
+  const int slots[1] = {S_KE};
+  block_write_row<1>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+}
+
+void launch_ke_rows(const SysView& s, const IntView& iv, hipStream_t st, double* rows) {
+  hipLaunchKernelGGL(k_ke_rows, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, iv, rows);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_v(SysView s, const double* scale) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.n) return;
+  double f = *scale;
+  s.vx[i] = f * s.vx[i];
+  s.vy[i] = f * s.vy[i];
+  s.vz[i] = f * s.vz[i];
+}
+
+void launch_scale_v(const SysView& s, hipStream_t st, const double* scale) {
+  hipLaunchKernelGGL(k_scale_v, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, scale);
+}
+
+// ------------------------------------------------------------------ NHC
+
+struct RowsDesc {
+  const double* rows;
+  int nrows;
+};
+
+__device__ void nhc_integrate(NhcState& S, const NhcParams& p) {
+  double T = S.tempT;
+  double kecur = p.tdof * p.boltz * T;
+  S.eta_mass[0] = p.tdof * p.boltz * S.t_target / (p.t_freq * p.t_freq);
+  for (int c = 1; c < 3; ++c) S.eta_mass[c] = p.boltz * S.t_target / (p.t_freq * p.t_freq);
+  S.eta_dotdot[0] = S.eta_mass[0] > 0.0 ? (kecur - S.ke_target) / S.eta_mass[0] : 0.0;
+  const double ncfac = 1.0;
+  double expfac;
+  for (int c = 2; c > 0; --c) {
+    expfac = exp(-ncfac * p.dt8 * S.eta_dot[c + 1]);
+    S.eta_dot[c] *= expfac;
+    S.eta_dot[c] += S.eta_dotdot[c] * ncfac * p.dt4;
+    S.eta_dot[c] *= p.tdrag;
+    S.eta_dot[c] *= expfac;
+  }
+  expfac = exp(-ncfac * p.dt8 * S.eta_dot[1]);
+  S.eta_dot[0] *= expfac;
+  S.eta_dot[0] += S.eta_dotdot[0] * ncfac * p.dt4;
+  S.eta_dot[0] *= p.tdrag;
+  S.eta_dot[0] *= expfac;
+  S.factor = exp(-ncfac * p.dthalf * S.eta_dot[0]);
+  S.tempT = T * S.factor * S.factor;
+  kecur = p.tdof * p.boltz * S.tempT;
+  S.eta_dotdot[0] = S.eta_mass[0] > 0.0 ? (kecur - S.ke_target) / S.eta_mass[0] : 0.0;
+  for (int c = 0; c < 3; ++c) S.eta[c] += ncfac * p.dthalf * S.eta_dot[c];
+  S.eta_dot[0] *= expfac;
+  S.eta_dot[0] += S.eta_dotdot[0] * ncfac * p.dt4;
+  S.eta_dot[0] *= expfac;
+  for (int c = 1; c < 3; ++c) {
+    expfac = exp(-ncfac * p.dt8 * S.eta_dot[c + 1]);
+    S.eta_dot[c] *= expfac;
+    S.eta_dotdot[c] = (S.eta_mass[c - 1] * S.eta_dot[c - 1] * S.eta_dot[c - 1] - p.boltz * S.t_target) / S.eta_mass[c];
+    S.eta_dot[c] += S.eta_dotdot[c] * ncfac * p.dt4;
+    S.eta_dot[c] *= expfac;
+  }
+}
+
+__device__ void temp_target(NhcState& S, const NhcParams& p) {
+  if (p.t_stop == p.t_start) {
+    S.t_target = p.t_start;
+  } else {
+    double delta = (double)S.step;
+    if (delta != 0.0) delta = delta / (double)p.num_steps;
+    S.t_target = p.t_start + delta * (p.t_stop - p.t_start);
+  }
+  S.ke_target = p.tdof * p.boltz * S.t_target;
+}
+
+__device__ void compute_tempe(NhcState& S, const NhcParams& p, double ke_sum) {
+  S.tempT_sum = ke_sum;
+  double n = p.n;
+  if (p.inbuild)
+    S.tempT = 0.5 * ke_sum / (3 * n / 2.0);
+  else if (p.fix_shake == 0)
+    S.tempT = 0.5 * ke_sum / ((3 * n - 3) * p.kB / 2.0);
+  else if (p.fix_shake == 1)
+    S.tempT = 0.5 * ke_sum / ((3 * n - n - 3) * p.kB / 2.0);
+  else
+    S.tempT = 0.5 * ke_sum / ((3 * n) * p.kB / 2.0);
+}
+
+__device__ void compute_pressure(NhcState& S, const NhcParams& p, const double* sums) {
+  for (int k = 0; k < 6; ++k) {
+    double w = sums[S_VIR_NEAR + k];
+    if (!p.lj_only) w += sums[S_VIR_K + k] * p.qqr2e;
+    if (p.cvff) {
+      w += sums[S_VIR_BOND + k] + sums[S_VIR_ANGLE + k];
+      if (p.shake_vir) w += sums[S_VIR_SHAKE + k];
+    }
+    S.virial[k] = w;
+  }
+  S.pressure = (p.tdof * p.boltz * S.tempT + S.virial[0] + S.virial[1] + S.virial[2]) / 3.0 * (1.0 / p.volume_range) *
+               p.nktv2p;
+  S.e_lj = sums[S_ELJ] / p.n;
+  S.e_coul = sums[S_ECOUL] / p.n;
+  S.e_bond = p.nb > 0 ? sums[S_EBOND] / p.nb : 0.0;
+  S.e_angle = p.na > 0 ? sums[S_EANGLE] / p.na : 0.0;
+}
+
+// One workgroup: reduce the partial rows (each wavefront owns a set of slots,
+// lanes stride the rows, fixed-order shuffle tree), then thread 0 advances the
+// thermostat state.
+__global__ __launch_bounds__(1024) void k_nhc(NhcState* state, NhcParams p, RowsDesc rd, int phase) {
+  __shared__ double sums[S_NSLOT];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int slot = wave; slot < S_NSLOT; slot += blockDim.x / 64) {
+    double acc = 0.0;
+    for (int r = lane; r < rd.nrows; r += 64) acc += rd.rows[(size_t)r * S_NSLOT + slot];
+    acc = wave_sum(acc);
+    if (lane == 0) sums[slot] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  NhcState S = *state;
+  switch (phase) {
+    case 3:  // SetUp: ComputeTempe, ComputeTempTarget, chain masses (ExecutionNVT.cpp:1106-1122)
+      compute_tempe(S, p, sums[S_KE]);
+      temp_target(S, p);
+      S.eta_mass[0] = p.tdof * p.boltz * S.t_target / (p.t_freq * p.t_freq);
+      for (int c = 1; c < 3; ++c) S.eta_mass[c] = p.boltz * S.t_target / (p.t_freq * p.t_freq);
+      for (int c = 1; c < 3; ++c)
+        S.eta_dotdot[c] = (S.eta_mass[c - 1] * S.eta_dot[c - 1] * S.eta_dot[c - 1] - p.boltz * S.t_target) / S.eta_mass[c];
+      compute_pressure(S, p, sums);
+      S.factor = 1.0;
+      break;
+    case 1:  // InitialIntegrate: TakeStep increments the step, then ComputeTempTarget + NHC
+      S.step += 1;
+      if (p.mode == 0) {
+        temp_target(S, p);
+        nhc_integrate(S, p);
+      } else {
+        S.factor = 1.0;
+      }
+      break;
+    case 2:  // FinalIntegrate: ComputeTempe, Compute_Pressure_Scalar, NHC
+      compute_tempe(S, p, sums[S_KE]);
+      compute_pressure(S, p, sums);
+      if (p.mode == 0)
+        nhc_integrate(S, p);
+      else
+        S.factor = 1.0;
+      break;
+    case 4: {  // Berendsen / rescale: ComputeTempe, rescale, pressure (ExecutionNVT.cpp:90-120, 270-291)
+      compute_tempe(S, p, sums[S_KE]);
+      double coeff;
+      if (p.mode == 2)
+        coeff = sqrt(p.t_start / S.tempT);
+      else
+        coeff = sqrt(1.0 + (p.dt / p.t_period) * (p.t_start / S.tempT - 1.0));
+      S.factor = coeff;
+      compute_pressure(S, p, sums);
+      break;
+    }
+    case 5:  // Berendsen-path step start: only the step counter
+      S.step += 1;
+      S.factor = 1.0;
+      break;
+    default:
+      break;
+  }
+  *state = S;
+}
+
+void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const double* rows, int nrows, int phase, hipStream_t st) {
+  RowsDesc rd{rows, nrows};
+  hipLaunchKernelGGL(k_nhc, dim3(1), dim3(1024), 0, st, st_dev, p, rd, phase);
+}
+
+}  // namespace rbmd

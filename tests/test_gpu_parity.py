@@ -1,0 +1,242 @@
+"""HIP path vs the CPU FP64 oracle on identical seeded inputs (FP64 kernels).
+
+Bars: cell lists and neighbour lists bit-exact (ids, order, image offsets);
+forces, virials and energies within 1e-10 relative (max over atoms of
+|dF_i| / max(|F_i|, rms F)); MD trajectories within 1e-8 after 20 steps.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from rbmd_amd.lib import GpuSystem
+from tests.systems import rel_err, salt, salt_run, water, water_run
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def pair(ctx, system, run):
+    print(f"[case] n={system.n_atoms} kind={system.kind} nbr={run.neighbor_type} ff={run.force_field}", flush=True)
+    return GpuSystem(ctx, system, run), oracle.OracleSystem(system, run)
+
+
+@pytest.mark.parametrize("kind", ["salt", "water"])
+def test_locator_bit_exact(ctx, kind):
+    s, r = (salt(), salt_run()) if kind == "salt" else (water(), water_run())
+    g, o = pair(ctx, s, r)
+    assert np.array_equal(g.geometry()[0], o.geometry()[0])
+    gc, gp, gl, gu = g.locator()
+    oc, op, ol, ou = o.locator()
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(gp, op)
+    assert np.array_equal(gl, ol)
+    assert np.array_equal(gu, ou)
+
+
+def test_locator_edges(ctx):
+    """atoms exactly at lo, at hi - ulp, at hi (wrapped by the caller), and a cell-boundary lattice"""
+    s = salt(n=512, seed=11)
+    x = s.x.copy()
+    L = s.hi[0]
+    x[0] = [0.0, 0.0, 0.0]
+    x[1] = [np.nextafter(L, 0), np.nextafter(L, 0), np.nextafter(L, 0)]
+    x[2] = [L / 2, L / 4, 3 * L / 4]
+    x[3] = [4.0, 8.0, 12.0]  # exact multiples of the cell width
+    s2 = type(s)(**{**s.__dict__, "x": x})
+    g, o = pair(ctx, s2, salt_run())
+    for a, b in zip(g.locator(), o.locator()):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["salt", "water"])
+def test_neighbor_lists_bit_exact(ctx, kind):
+    s, r = (salt(), salt_run()) if kind == "salt" else (water(), water_run())
+    g, o = pair(ctx, s, r)
+    cap = 600 if kind == "water" else 400
+    gi, go, gn = g.neighbors(cap)
+    oi, oo, on = o.neighbors(cap)
+    assert np.array_equal(gn, on)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(go, oo)
+
+
+def test_neighbor_counts_bruteforce(ctx):
+    s = salt(n=1000, seed=7)
+    g, o = pair(ctx, s, salt_run())
+    _, _, gn = g.neighbors(400)
+    assert np.array_equal(gn, o.bruteforce_counts(4.0))
+
+
+@pytest.mark.parametrize("kind", ["salt", "water"])
+def test_real_space_force(ctx, kind):
+    s, r = (salt(), salt_run()) if kind == "salt" else (water(), water_run())
+    g, o = pair(ctx, s, r)
+    g.neighbors(0)
+    o.neighbors(0)
+    gf, gv, ge, gc = g.force_real()
+    of, ov, oe, oc = o.force_real()
+    assert rel_err(gf, of) < FP64_TOL
+    assert rel_err(gv, ov) < FP64_TOL
+    assert abs(ge.sum() - oe.sum()) <= FP64_TOL * abs(oe).sum()
+    assert abs(gc.sum() - oc.sum()) <= FP64_TOL * abs(oc).sum()
+
+
+def test_lj_only_force(ctx):
+    s = salt(n=2000, seed=9)
+    r = salt_run(force_field="LJ/CUT", coulomb_type="NONE", cut_off=2.5, r_core=2.5)
+    g, o = pair(ctx, s, r)
+    g.neighbors(0)
+    o.neighbors(0)
+    gf, gv, ge, _ = g.force_real()
+    of, ov, oe, _ = o.force_real()
+    assert rel_err(gf, of) < FP64_TOL
+    assert rel_err(gv, ov) < FP64_TOL
+
+
+def test_sampler_bit_exact():
+    box = np.array([20.0, 20.0, 20.0])
+    for stream in (0, 1, 17, 4096):
+        a = oracle.rbe_sample(0.64, box, 100, 12345, stream)
+        from rbmd_amd.lib import rbe_sample_host
+
+        b = rbe_sample_host(0.64, box, 100, 12345, stream)
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["salt", "water"])
+def test_rbe_structure_factor_and_force(ctx, kind):
+    s, r = (salt(), salt_run()) if kind == "salt" else (water(), water_run())
+    g, o = pair(ctx, s, r)
+    m = oracle.rbe_sample(r.alpha, s.box, r.rbe_p, 777, 3)
+    g.set_kvectors(m)
+    o.set_kvectors(m)
+    grho = g.structure_factor(r.rbe_p)
+    orho = o.structure_factor(r.rbe_p)
+    scale = np.abs(orho).max()
+    assert np.abs(grho - orho).max() < FP64_TOL * scale
+    gf, gv = g.rbe_force()
+    of, ov = o.rbe_force()
+    assert rel_err(gf, of) < FP64_TOL
+    assert rel_err(gv, ov) < 1e-9
+
+
+def test_ewald_force_and_madelung(ctx):
+    from rbmd_amd import RunConfig, rocksalt_lattice
+
+    lat = rocksalt_lattice(8)
+    beta = 1.3
+    run = RunConfig(unit="LJ", force_field="LJ/CUT/COUL/LONG", cut_off=3.99, r_core=3.99, coulomb_type="EWALD",
+                    alpha=beta * beta, kmax=[10, 10, 10], temperature=[1, 1, 1])
+    g, o = pair(ctx, lat, run)
+    gf, gv, ge = g.ewald_force([10, 10, 10])
+    of, ov, oe = o.ewald_force([10, 10, 10])
+    assert abs(ge - oe) < 1e-10 * abs(oe)
+    # perfect lattice: all forces vanish
+    assert np.abs(gf).max() < 1e-10
+    # Madelung constant from real + k space + self on the GPU (k grid converged at 22)
+    g2, _ = pair(ctx, lat, RunConfig(**{**run.__dict__, "kmax": [22, 22, 22]}))
+    g2.neighbors(0)
+    _, _, _, ec = g2.force_real()
+    _, _, ek = g2.ewald_force([22, 22, 22])
+    self_e = -np.sqrt(run.alpha / np.pi) * np.sum(lat.charge ** 2)
+    E = ec.sum() + ek + self_e
+    assert abs(-2 * E / lat.n_atoms - 1.747564594633) < 1e-10
+
+
+def test_bonded(ctx):
+    # molecules are generated exactly at r0 / theta0; jitter them so the terms are nonzero
+    s = water()
+    rng = np.random.default_rng(2)
+    s = type(s)(**{**s.__dict__, "x": s.x + rng.uniform(-0.05, 0.05, s.x.shape)})
+    g, o = pair(ctx, s, water_run())
+    gf, gb, ga = g.force_bonded()
+    of, ob, oa = o.force_bonded()
+    assert rel_err(gf, of) < FP64_TOL
+    assert abs(gb - ob) <= FP64_TOL * abs(ob) + 1e-14
+    assert abs(ga - oa) <= FP64_TOL * abs(oa) + 1e-14
+
+
+def _md_compare(ctx, system, run, steps, tol):
+    g, o = pair(ctx, system, run)
+    g.md_init()
+    assert o.md_init() == 0
+    xg, vg, fg, _ = g.state()
+    xo, vo, fo, _ = o.state()
+    assert rel_err(fg, fo) < FP64_TOL
+    g.md_step(steps)
+    assert o.md_step(steps) == 0
+    xg, vg, fg, ig = g.state()
+    xo, vo, fo, io = o.state()
+    box = system.box
+    dx = xg - xo
+    dx -= box * np.round(dx / box)
+    assert np.abs(dx).max() < tol * box.max()
+    assert rel_err(vg, vo) < tol
+    assert rel_err(fg, fo) < tol
+    tg, to = g.thermo(), o.thermo()
+    assert tg["step"] == to["step"] == steps
+    for k in ("temperature", "pressure", "e_lj", "e_coul", "e_bond", "e_angle"):
+        assert abs(tg[k] - to[k]) <= tol * max(1.0, abs(to[k])), (k, tg[k], to[k])
+    for k in range(3):
+        assert abs(tg["nhc_eta"][k] - to["nhc_eta"][k]) <= tol * max(1.0, abs(to["nhc_eta"][k]))
+    return tg
+
+
+def test_md_salt_nvt(ctx):
+    _md_compare(ctx, salt(), salt_run(), 20, 1e-8)
+
+
+def test_md_water_nvt_shake(ctx):
+    t = _md_compare(ctx, water(), water_run(), 20, 1e-8)
+    assert t["status_flags"] == 0
+
+
+def test_md_salt_berendsen(ctx):
+    _md_compare(ctx, salt(n=1000, seed=4), salt_run(temp_ctrl="BERENDSEN", temperature=[1.5, 1.5, 0.1]), 10, 1e-8)
+
+
+def test_md_water_nve(ctx):
+    _md_compare(ctx, water(), water_run(temp_ctrl="NONE"), 10, 1e-8)
+
+
+def test_rbl_lists_and_force(ctx):
+    s = salt(n=4000, seed=3)
+    r = salt_run(neighbor_type="RBL", r_core=2.0, neighbor_sample_num=40, erfc_mode="analytic")
+    g, o = pair(ctx, s, r)
+    cap = 300
+    gi, go, gn = g.neighbors(cap)
+    oi, oo, on = o.neighbors(cap)
+    assert np.array_equal(gn, on)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(go, oo)
+    gf, gv, _, _ = g.force_real()
+    of, ov, _, _ = o.force_real()
+    assert rel_err(gf, of) < FP64_TOL
+    assert rel_err(gv, ov) < 1e-9
+
+
+def test_rbl_table_mode(ctx):
+    s = salt(n=4000, seed=3)
+    r = salt_run(neighbor_type="RBL", r_core=2.0, neighbor_sample_num=40, erfc_mode="table")
+    g, o = pair(ctx, s, r)
+    g.neighbors(300)
+    o.neighbors(300)
+    gf, _, _, _ = g.force_real()
+    of, _, _, _ = o.force_real()
+    assert rel_err(gf, of) < FP64_TOL
+
+
+def test_error_paths(ctx):
+    from rbmd_amd.lib import RbmdError
+
+    s = salt(n=512, seed=1)
+    r = salt_run()
+    g = GpuSystem(ctx, s, r)
+    with pytest.raises(RbmdError):
+        g.md_step(1)  # before md_init
+    x = s.x.copy()
+    x[5, 0] = np.nan
+    g.set_state(x, s.v)
+    with pytest.raises(RbmdError):
+        g.md_init()
