@@ -102,6 +102,8 @@ class Thermo(C.Structure):
         ("energy_step", C.c_int64),
         ("shake_max_iter", C.c_int32),
         ("status_flags", C.c_int32),
+        ("misfit_atoms", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     def as_dict(self):

@@ -161,6 +161,7 @@ struct rbmd_sys {
   double* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   int ring_R = 256;
+  int export_cap = 0;
   int64_t ring_batch = -1;  // batch of steps currently staged in half (batch % 2)
   int64_t ring_have[2] = {-1, -1};
   bool injected = false;
@@ -238,20 +239,42 @@ void rbl_capacity(const rbmd_params& p, int n, double volume, int* rs_num, int* 
   *rs_num = (int)(2 * rsn + 32);
 }
 
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Default list capacity.  VERLET: each of the 3 x-slab segments holds the mean
+// full-sphere count + 64 (a slab holds at most ~70% of the sphere);
+// RBL: the core region plus twice the expected kept shell atoms + 64.
 int default_cap(const rbmd_sys* s) {
   double rho = s->n / s->v.volume;
   double rc = s->p.cut_off;
   double c = rho * 4.0 / 3.0 * kPi * rc * rc * rc;
-  int cap = (int)(2.0 * c + 64);
-  if (s->v.rbl) cap = s->v.rs_num + (int)(2.0 * c / s->v.pice_num + 64);
   int st = (2 * s->v.nc + 1);
   long maxc = (long)std::max(s->n, 1) * st * st * st;
-  return (int)std::min<long>(cap, maxc);
+  if (s->v.rbl) return round_up((int)std::min<long>(s->v.rs_num + (int)(2.0 * c / s->v.pice_num + 64), maxc), kLanes);
+  return (int)std::min<long>((long)(c + 64), maxc);
 }
 
+// cap: the width the caller exports (0: default).  VERLET keeps one segment of
+// round_up(cap) slots per x-slab, so no segment can overflow below cap.
 rbmd_status ensure_nbr(rbmd_sys* s, int cap) {
-  if (cap <= 0) cap = default_cap(s);
-  if (s->v.nbr && cap == s->v.cap) return RBMD_OK;
+  int seg_cap, nseg, export_cap;
+  if (s->v.rbl) {
+    nseg = 1;
+    export_cap = cap > 0 ? cap : default_cap(s);
+    seg_cap = round_up(export_cap, kLanes);
+    if (seg_cap < s->v.rs_num + 1) return fail(RBMD_ERR_ARG, "RBL capacity %d below the core region %d", cap, s->v.rs_num);
+  } else {
+    nseg = 3;
+    if (cap > 0) {
+      seg_cap = round_up(cap, kLanes);
+      export_cap = cap;
+    } else {
+      seg_cap = round_up(default_cap(s), kLanes);
+      export_cap = 3 * seg_cap;
+    }
+  }
+  s->export_cap = export_cap;
+  if (s->v.nbr && seg_cap == s->v.seg_cap && nseg == s->v.nseg) return RBMD_OK;
   if (s->v.nbr) {
     auto it = std::find(s->allocs.begin(), s->allocs.end(), (void*)s->v.nbr);
     if (it != s->allocs.end()) s->allocs.erase(it);
@@ -259,9 +282,10 @@ rbmd_status ensure_nbr(rbmd_sys* s, int cap) {
     (void)hipFree(s->v.nbr);
     s->v.nbr = nullptr;
   }
-  if (s->v.rbl && cap < s->v.rs_num + 1) return fail(RBMD_ERR_ARG, "RBL capacity %d below rs_num %d", cap, s->v.rs_num);
-  s->v.cap = cap;
-  return s->dalloc(&s->v.nbr, (size_t)cap * s->n);
+  s->v.nseg = nseg;
+  s->v.seg_cap = seg_cap;
+  s->v.cap = nseg * seg_cap;
+  return s->dalloc(&s->v.nbr, (size_t)s->v.cap * s->n);
 }
 
 rbmd_status read_status(rbmd_sys* s, int* st) {
@@ -291,6 +315,7 @@ void gather_and_list(rbmd_sys* s) {
   rbmd_ctx* c = s->ctx;
   c->run("gather", [&] { launch_gather(s->v, c->stream); });
   c->run("nbr_build", [&] { launch_nbr_build(s->v, c->stream); });
+  c->run("nbr_misfit", [&] { launch_nbr_misfit(s->v, c->stream); });
   s->locator_built = true;
   s->nbr_built = true;
 }
@@ -560,6 +585,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   v.weighted = (have_sp && p->special_bonds) ? 1 : 0;
   if (v.rbl) rbl_capacity(*p, n, v.volume, &v.rs_num, &v.pice_num);
   v.n_items = v.ncell + n / kChunk + 1;
+  v.lds_bins = v.ncell <= 4096 ? 1 : 0;
 
   rbmd_status rc = RBMD_OK;
   auto A = [&](rbmd_status r) {
@@ -599,9 +625,11 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   A(s->dalloc(&v.items, v.n_items));
   A(s->dalloc(&v.n_items_dev, 1));
   A(s->dalloc(&v.misfit, n));
+  A(s->dalloc(&v.misfit_list, n));
+  A(s->dalloc(&v.misfit_count, 1));
   A(s->dalloc(&v.spos, n));
   A(s->dalloc(&v.stype, n));
-  A(s->dalloc(&v.nbr_num, 2 * (size_t)n));
+  A(s->dalloc(&v.nbr_num, 3 * (size_t)n));
   A(s->dalloc(&v.vir_near, 6 * (size_t)n));
   A(s->dalloc(&v.e_lj, n));
   A(s->dalloc(&v.e_coul, n));
@@ -711,13 +739,14 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   }
   // reduction rows
   int nblk = (n + kBlock - 1) / kBlock;
+  int pblk = (n * kLanes + kBlock - 1) / kBlock;  // pair kernel: kLanes threads per atom
   int sblk = (std::max(s->nmol, 1) + kBlock - 1) / kBlock;
   s->row_pair = 0;
-  s->row_k = nblk;
-  s->row_bond = 2 * nblk;
-  s->row_ke = 3 * nblk;
-  s->row_shake = 4 * nblk;
-  s->nrows = 4 * nblk + sblk;
+  s->row_k = pblk;
+  s->row_bond = pblk + nblk;
+  s->row_ke = pblk + 2 * nblk;
+  s->row_shake = pblk + 3 * nblk;
+  s->nrows = pblk + 3 * nblk + sblk;
   A(s->dalloc(&s->rows, (size_t)s->nrows * S_NSLOT));
   A(s->dalloc(&s->nhc, 1));
   A(s->dalloc(&s->shake_iter, 1));
@@ -928,6 +957,7 @@ rbmd_status rbmd_neighbors_build(rbmd_sys* s, int32_t cap) {
   if (!s->locator_built) RC_TRY(rbmd_locator_build(s));
   RC_TRY(ensure_nbr(s, cap));
   s->ctx->run("nbr_build", [&] { launch_nbr_build(s->v, s->ctx->stream); });
+  s->ctx->run("nbr_misfit", [&] { launch_nbr_misfit(s->v, s->ctx->stream); });
   s->nbr_built = true;
   return check_launch("neighbors_build");
 }
@@ -935,10 +965,10 @@ rbmd_status rbmd_neighbors_build(rbmd_sys* s, int32_t cap) {
 rbmd_status rbmd_neighbors_export(rbmd_sys* s, int32_t cap, int32_t* ids, double* offsets, int32_t* num) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
   if (!s->nbr_built) return fail(RBMD_ERR_STATE, "neighbour lists not built");
-  if (cap != s->v.cap) return fail(RBMD_ERR_ARG, "cap %d differs from the built capacity %d", cap, s->v.cap);
+  if (cap != s->export_cap) return fail(RBMD_ERR_ARG, "cap %d differs from the built width %d", cap, s->export_cap);
   int n = s->n;
   hipStream_t st = s->ctx->stream;
-  std::vector<int> nbr((size_t)cap * n), cnt(2 * (size_t)n), pid(n), rank(n);
+  std::vector<int> nbr((size_t)s->v.cap * n), cnt(3 * (size_t)n), pid(n), rank(n);
   HIP_TRY(hipMemcpyAsync(nbr.data(), s->v.nbr, nbr.size() * 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(cnt.data(), s->v.nbr_num, cnt.size() * 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(pid.data(), s->v.point_ids, n * 4, hipMemcpyDeviceToHost, st));
@@ -946,48 +976,55 @@ rbmd_status rbmd_neighbors_export(rbmd_sys* s, int32_t cap, int32_t* ids, double
   HIP_TRY(hipStreamSynchronize(st));
   int stt = 0;
   RC_TRY(read_status(s, &stt));
-  const bool rbl = s->v.rbl;
+  bool over = false;
   for (int a = 0; a < n; ++a) {
     int i = rank[a];
-    auto emit = [&](int slot) {
-      int e = nbr[(size_t)slot * n + i];
-      int js = e >> kShiftBits;
-      int code = e & ((1 << kShiftBits) - 1);
-      int sh[3] = {(code & 7) - 3, ((code >> 3) & 7) - 3, ((code >> 6) & 7) - 3};
-      if (ids) ids[(size_t)a * cap + slot] = pid[js];
-      if (offsets)
-        for (int d = 0; d < 3; ++d)
-          offsets[((size_t)a * cap + slot) * 3 + d] = s->v.dxdydz[d] * (double)(sh[d] * s->v.dims[d]);
-    };
     if (ids)
       for (int k = 0; k < cap; ++k) ids[(size_t)a * cap + k] = -1;
     if (offsets)
       for (int k = 0; k < 3 * cap; ++k) offsets[(size_t)a * cap * 3 + k] = 0.0;
-    if (!rbl) {
-      for (int k = 0; k < cnt[i]; ++k) emit(k);
-      if (num) num[a] = cnt[i];
+    auto emit = [&](int out_slot, int slot) {
+      if (out_slot >= cap) {
+        over = true;
+        return;
+      }
+      int e = nbr[nbr_index(slot, i, n)];
+      int js = e >> kShiftBits;
+      int code = e & ((1 << kShiftBits) - 1);
+      int sh[3] = {(code & 7) - 3, ((code >> 3) & 7) - 3, ((code >> 6) & 7) - 3};
+      if (ids) ids[(size_t)a * cap + out_slot] = pid[js];
+      if (offsets)
+        for (int d = 0; d < 3; ++d)
+          offsets[((size_t)a * cap + out_slot) * 3 + d] = s->v.dxdydz[d] * (double)(sh[d] * s->v.dims[d]);
+    };
+    if (!s->v.rbl) {
+      int out = 0;
+      for (int g = 0; g < s->v.nseg; ++g)
+        for (int k = 0; k < cnt[(size_t)g * n + i]; ++k) emit(out++, g * s->v.seg_cap + k);
+      if (num) num[a] = out;
     } else {
-      for (int k = 0; k < cnt[2 * i]; ++k) emit(k);
-      for (int k = 0; k < cnt[2 * i + 1]; ++k) emit(s->v.rs_num + k);
+      for (int k = 0; k < cnt[2 * i]; ++k) emit(k, k);
+      for (int k = 0; k < cnt[2 * i + 1]; ++k) emit(s->v.rs_num + k, s->v.rs_num + k);
       if (num) {
         num[2 * a] = cnt[2 * i];
         num[2 * a + 1] = cnt[2 * i + 1];
       }
     }
   }
+  if (over) return fail(RBMD_ERR_CAPACITY, "a list is longer than the export width %d", cap);
   return status_to_rc(stt);
 }
 
 rbmd_status rbmd_neighbors_capacity(const rbmd_sys* s, int32_t* cap) {
   if (!s || !cap) return fail(RBMD_ERR_ARG, "bad arguments");
-  *cap = s->v.cap;
+  *cap = s->export_cap;
   return RBMD_OK;
 }
 
 rbmd_status rbmd_neighbors_count(rbmd_sys* s, int64_t* entries) {
   if (!s || !entries) return fail(RBMD_ERR_ARG, "bad arguments");
   if (!s->nbr_built) return fail(RBMD_ERR_STATE, "neighbour lists not built");
-  int m = s->v.rbl ? 2 * s->n : s->n;
+  int m = s->v.rbl ? 2 * s->n : s->v.nseg * s->n;
   std::vector<int> cnt(m);
   HIP_TRY(hipMemcpyAsync(cnt.data(), s->v.nbr_num, (size_t)m * 4, hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipStreamSynchronize(s->ctx->stream));
@@ -1279,12 +1316,14 @@ rbmd_status rbmd_md_run_graph(rbmd_sys* s, int64_t n_steps) { return rbmd_md_ste
 rbmd_status rbmd_md_thermo(rbmd_sys* s, rbmd_thermo* o) {
   if (!s || !o) return fail(RBMD_ERR_ARG, "bad arguments");
   NhcState S;
-  int stt = 0, it = 0;
+  int stt = 0, it = 0, mf = 0;
+  HIP_TRY(hipMemcpyAsync(&mf, s->v.misfit_count, 4, hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipMemcpyAsync(&S, s->nhc, sizeof(S), hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipMemcpyAsync(&stt, s->v.status, 4, hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipMemcpyAsync(&it, s->shake_iter, 4, hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipStreamSynchronize(s->ctx->stream));
   fill_thermo(s, S, stt, it, o);
+  o->misfit_atoms = mf;
   return RBMD_OK;
 }
 

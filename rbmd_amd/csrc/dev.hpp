@@ -12,6 +12,14 @@ constexpr int kWave = 64;       // CDNA wavefront width
 constexpr int kChunk = 64;      // i-atoms per neighbour-build work item (one wavefront)
 constexpr int kMaxSpecial = 8;  // special-list entries held in registers per atom
 constexpr int kShiftBits = 9;   // 3 bits per axis image shift in packed neighbour entries
+constexpr int kLanes = 4;       // lanes sharing one atom's neighbour list in the pair kernel
+
+// Neighbour-list slot layout: slot k of atom i (sorted order) lives at
+// ((k / kLanes) * n + i) * kLanes + k % kLanes, so the kLanes lanes that share
+// atom i, and consecutive atoms, read consecutive words.
+__host__ __device__ __forceinline__ size_t nbr_index(int slot, int i, int n) {
+  return ((size_t)(slot / kLanes) * n + i) * kLanes + (slot % kLanes);
+}
 
 // Everything a kernel needs to know about one system.  Passed by value.
 struct SysView {
@@ -20,7 +28,9 @@ struct SysView {
   int ncell;        // dims0*dims1*dims2
   int dims[3];
   int nc;           // stencil half width (ceil(rc / rs))
-  int cap;          // neighbour-list capacity per atom
+  int cap;          // neighbour-list slots per atom (VERLET: nseg * seg_cap)
+  int nseg;         // VERLET: one list segment per x-slab of the stencil (2nc+1); RBL: 1
+  int seg_cap;      // slots per segment (multiple of kLanes)
   int n_items;      // capacity of the work-item list
   int weighted;     // special-bond weights in the pair term
   int lj_only;      // LJ/CUT force field
@@ -49,12 +59,15 @@ struct SysView {
   int *items;       // [n_items] packed (cell << 8 | chunk)
   int *n_items_dev; // [1]
   int *misfit;      // [n] sorted position flag
+  int *misfit_list; // [n] sorted positions of misfit atoms
+  int *misfit_count;// [1]
   // sorted copies for the pair kernels
   double4* spos;    // [n] x,y,z,q in sorted order
   int* stype;       // [n]
   // neighbour lists, column-major [cap][n] of packed (j_sorted << 9 | shift code)
-  int* nbr;
-  int* nbr_num;     // [n] (VERLET) or [2n] (RBL: core, shell)
+  int* nbr;         // layout nbr_index(slot, i, n)
+  int lds_bins;     // binning kernels aggregate counts in LDS (ncell small enough)
+  int* nbr_num;     // VERLET [nseg][n] counts per segment; RBL [2n] (core, shell kept)
   // per-atom force components (original order)
   double *fnx, *fny, *fnz;  // real space, sorted order
   double *fkx, *fky, *fkz;  // k space
@@ -90,6 +103,7 @@ void launch_scatter(const SysView& s, hipStream_t st);
 void launch_cell_sort(const SysView& s, hipStream_t st);
 void launch_gather(const SysView& s, hipStream_t st);
 void launch_nbr_build(const SysView& s, hipStream_t st);
+void launch_nbr_misfit(const SysView& s, hipStream_t st);
 void launch_fill_int(int* p, int v, int n, hipStream_t st);
 void launch_fill_double(double* p, double v, int n, hipStream_t st);
 

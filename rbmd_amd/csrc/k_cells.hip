@@ -16,6 +16,7 @@
 // broadcast to all lanes.  Lists are column-major [slot][n] so that the pair
 // kernel's per-lane reads of slot k are coalesced.
 #include "dev.hpp"
+#include "reduce.hpp"
 
 namespace rbmd {
 
@@ -37,16 +38,6 @@ void launch_fill_double(double* p, double v, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_double, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, p, v, n);
 }
 
-// BinPointsWorklet: ijk = trunc((x - min) / dxdydz), clamped to [0, dims-1]
-__device__ __forceinline__ int bin_axis(double x, double lo, double d, int dims) {
-  double t = (x - lo) / d;
-  int k = 0;
-  if (t == t) k = (int)fmax(fmin(t, 2.0e9), -2.0e9);  // trunc toward zero like the Vec3f->Id3 cast
-  k = k < 0 ? 0 : k;
-  k = k > dims - 1 ? dims - 1 : k;
-  return k;
-}
-
 // InWhichCell (ExecPointLocator.h:271-274): unclamped trunc
 __device__ __forceinline__ int cell_axis(double x, double lo, double d) {
   double t = (x - lo) / d;
@@ -56,20 +47,25 @@ __device__ __forceinline__ int cell_axis(double x, double lo, double d) {
 
 __global__ __launch_bounds__(kBlock) void k_bin(SysView s, const double* __restrict__ x, const double* __restrict__ y,
                                                 const double* __restrict__ z) {
+  extern __shared__ int lds_cells[];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n) return;
-  double px = x[i], py = y[i], pz = z[i];
-  if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) atomicOr(s.status, 1);
-  int a = bin_axis(px, s.lo[0], s.dxdydz[0], s.dims[0]);
-  int b = bin_axis(py, s.lo[1], s.dxdydz[1], s.dims[1]);
-  int c = bin_axis(pz, s.lo[2], s.dxdydz[2], s.dims[2]);
-  int cell = a + b * s.dims[0] + c * s.dims[0] * s.dims[1];
-  s.cell_of[i] = cell;
-  atomicAdd(&s.cell_count[cell], 1);
+  int cell = -1;
+  if (i < s.n) {
+    double px = x[i], py = y[i], pz = z[i];
+    if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) atomicOr(s.status, 1);
+    int a = bin_axis(px, s.lo[0], s.dxdydz[0], s.dims[0]);
+    int b = bin_axis(py, s.lo[1], s.dxdydz[1], s.dims[1]);
+    int c = bin_axis(pz, s.lo[2], s.dxdydz[2], s.dims[2]);
+    cell = a + b * s.dims[0] + c * s.dims[0] * s.dims[1];
+    s.cell_of[i] = cell;
+  }
+  count_cells(s, cell, lds_cells);
 }
 
+size_t bin_lds_bytes(const SysView& s) { return s.lds_bins ? (size_t)s.ncell * sizeof(int) : 0; }
+
 void launch_bin(const SysView& s, hipStream_t st, const double* x, const double* y, const double* z) {
-  hipLaunchKernelGGL(k_bin, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, x, y, z);
+  hipLaunchKernelGGL(k_bin, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), bin_lds_bytes(s), st, s, x, y, z);
 }
 
 // block-wide exclusive scan of one int per thread (1024 threads = 16 waves)
@@ -127,6 +123,7 @@ __global__ __launch_bounds__(1024) void k_scan(SysView s) {
     icarry += itot;
   }
   if (threadIdx.x == 0) {
+    s.misfit_count[0] = 0;
     s.cell_start[s.ncell] = carry;
     s.n_items_dev[0] = icarry < s.n_items ? icarry : s.n_items;
     if (icarry > s.n_items) atomicOr(s.status, 4);
@@ -137,15 +134,34 @@ void launch_scan(const SysView& s, hipStream_t st) {
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s);
 }
 
+// Scatter atom ids into their cells.  With LDS aggregation each workgroup
+// reserves one range per cell with a single global atomic and places its atoms
+// by their LDS rank; the order inside a cell is fixed afterwards by k_cell_sort.
 __global__ __launch_bounds__(kBlock) void k_scatter(SysView s) {
+  extern __shared__ int lds[];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n) return;
-  int pos = atomicAdd(&s.cursor[s.cell_of[i]], 1);
-  s.point_ids[pos] = i;
+  int cell = i < s.n ? s.cell_of[i] : -1;
+  if (!s.lds_bins) {
+    if (cell >= 0) s.point_ids[atomicAdd(&s.cursor[cell], 1)] = i;
+    return;
+  }
+  int* cnt = lds;
+  int* base = lds + s.ncell;
+  for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) cnt[c] = 0;
+  __syncthreads();
+  int local = cell >= 0 ? atomicAdd(&cnt[cell], 1) : 0;
+  __syncthreads();
+  for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) {
+    int v = cnt[c];
+    base[c] = v ? atomicAdd(&s.cursor[c], v) : 0;
+  }
+  __syncthreads();
+  if (cell >= 0) s.point_ids[base[cell] + local] = i;
 }
 
 void launch_scatter(const SysView& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_scatter, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
+  size_t lds = s.lds_bins ? 2 * (size_t)s.ncell * sizeof(int) : 0;
+  hipLaunchKernelGGL(k_scatter, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), lds, st, s);
 }
 
 constexpr int kSortCap = 4096;
@@ -192,7 +208,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(SysView s) {
   int ua = cell_axis(px, s.lo[0], s.dxdydz[0]);
   int ub = cell_axis(py, s.lo[1], s.dxdydz[1]);
   int uc = cell_axis(pz, s.lo[2], s.dxdydz[2]);
-  s.misfit[p] = (ua != a || ub != b || uc != c) ? 1 : 0;
+  const int mf = (ua != a || ub != b || uc != c) ? 1 : 0;
+  s.misfit[p] = mf;
+  if (mf) s.misfit_list[atomicAdd(s.misfit_count, 1)] = p;
 }
 
 void launch_gather(const SysView& s, hipStream_t st) {
@@ -205,58 +223,267 @@ __device__ __forceinline__ int pack_entry(int j, int sa, int sb, int sc) {
   return (j << kShiftBits) | ((sa + 3) | ((sb + 3) << 3) | ((sc + 3) << 6));
 }
 
-// Record one accepted candidate for lane i.  VERLET: slot = count.  RBL: core
-// slots [0, rs_num), kept shell slots from rs_num (every pice_num-th shell atom).
-struct ListState {
-  int cnt, core, shell_seen, shell_kept;
+// wavefront-local LDS ordering (writes of all lanes visible before reads)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One stencil cell seen from cell (ca,cb,cc) at offset (a,b,e): wrapped cell,
+// image offset dxdydz * (wrapped - unwrapped) (PeriodicIndexOffset /
+// PeriodicCoordOffset, ExecPointLocator.h:298-319) and the packed shift code.
+struct StencilCell {
+  int js, je, sa, sb, sc;
+  double ox, oy, oz;
+  bool ok;
 };
 
-template <bool RBL>
-__device__ __forceinline__ void consider(const SysView& s, ListState& L, int i, int entry, double d2) {
-  if (!RBL) {
-    if (s.cut2 - d2 > 0.0001 && d2 > 0.0001) {
-      if (L.cnt < s.cap)
-        s.nbr[(size_t)L.cnt * s.n + i] = entry;
-      else
-        atomicOr(s.status, 4);
-      L.cnt++;
-    }
-  } else {
-    if (s.cut2 - d2 > 0.0001 && d2 - s.rs2 > 0.0001) {
-      if (L.shell_seen % s.pice_num == 0) {
-        int slot = s.rs_num + L.shell_kept;
-        if (slot < s.cap)
-          s.nbr[(size_t)slot * s.n + i] = entry;
-        else
-          atomicOr(s.status, 4);
-        L.shell_kept++;
+__device__ __forceinline__ StencilCell stencil_cell(const SysView& s, int ca, int cb, int cc, int a, int b, int e) {
+  StencilCell t;
+  int na = ca + a, nb = cb + b, ne = cc + e;
+  int wa = wrap_axis(na, s.dims[0]), wb = wrap_axis(nb, s.dims[1]), we = wrap_axis(ne, s.dims[2]);
+  t.ok = wa >= 0 && wb >= 0 && we >= 0;
+  if (!t.ok) {
+    t.js = t.je = 0;
+    return t;
+  }
+  t.ox = s.dxdydz[0] * (double)(wa - na);
+  t.oy = s.dxdydz[1] * (double)(wb - nb);
+  t.oz = s.dxdydz[2] * (double)(we - ne);
+  t.sa = (wa - na) / s.dims[0];
+  t.sb = (wb - nb) / s.dims[1];
+  t.sc = (we - ne) / s.dims[2];
+  if (t.sa < -3 || t.sa > 3 || t.sb < -3 || t.sb > 3 || t.sc < -3 || t.sc > 3) {
+    t.ok = false;
+    t.js = t.je = 0;
+    return t;
+  }
+  int wcell = wa + wb * s.dims[0] + we * s.dims[0] * s.dims[1];
+  t.js = s.cell_start[wcell];
+  t.je = s.cell_start[wcell + 1];
+  return t;
+}
+
+// VERLET list build.  One workgroup per work item (64 i-atoms of one cell),
+// one wavefront per x-slab a of the 3x3x3 stencil; wavefront a writes list
+// segment a + 1, so the concatenated segments are in the reference's order
+// (x offset outermost, then y, then z, then the cell's sorted order).
+//
+// The 9 cells of a slab form one candidate stream.  Each round, every lane
+// stages one candidate (coalesced load, image shift applied); candidates that
+// are farther than the cutoff from the i-chunk's bounding box are dropped by a
+// wave-uniform ballot and the rest are compacted into LDS.  Every lane then
+// tests the compacted candidates against its own i-atom with the reference's
+// FP64 predicate in a branch-free loop that only sets bits of an accept mask,
+// and finally writes the accepted entries in stream order.
+constexpr int kStageRounds = 1;
+
+__global__ __launch_bounds__(3 * kWave) void k_nbr_build_verlet(SysView s) {
+  __shared__ double4 stage[3][kWave];
+  __shared__ int stage_e[3][kWave];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int item_idx = blockIdx.x;
+  if (item_idx >= s.n_items_dev[0]) return;
+  const int item = __builtin_amdgcn_readfirstlane(s.items[item_idx]);
+  const int c = item >> 8;
+  const int chunk = item & 255;
+  const int start = s.cell_start[c];
+  const int cnt_c = s.cell_start[c + 1] - start;
+  const int local = chunk * kChunk + lane;
+  const int i = start + local;
+  bool active = local < cnt_c;
+  double4 pi = make_double4(0, 0, 0, 0);
+  if (active) {
+    pi = s.spos[i];
+    if (s.misfit[i]) active = false;  // handled by k_nbr_misfit_verlet
+  }
+  // bounding box of the active i-atoms (wave reductions)
+  double bl[3], bh[3];
+  {
+    double v[3] = {pi.x, pi.y, pi.z};
+    for (int d = 0; d < 3; ++d) {
+      double lo = active ? v[d] : 1e300, hi = active ? v[d] : -1e300;
+      for (int off = 1; off < kWave; off <<= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, kWave));
+        hi = fmax(hi, __shfl_xor(hi, off, kWave));
       }
-      L.shell_seen++;
-    } else if (s.rs2 - d2 > 0.0001 && d2 > 0.0001) {
-      if (L.core < s.rs_num)
-        s.nbr[(size_t)L.core * s.n + i] = entry;
+      bl[d] = lo;
+      bh[d] = hi;
+    }
+  }
+  const int ca = c % s.dims[0];
+  const int cb = (c / s.dims[0]) % s.dims[1];
+  const int cc = c / (s.dims[0] * s.dims[1]);
+  const int a = wave - 1;
+  const int seg0 = wave * s.seg_cap;
+  // the slab's 9 cells as one stream: per-cell start, count prefix, offset, code
+  int cs[9], pre[10], code[9];
+  double ofx[9], ofy[9], ofz[9];
+  pre[0] = 0;
+  for (int q = 0; q < 9; ++q) {
+    StencilCell t = stencil_cell(s, ca, cb, cc, a, q / 3 - 1, q % 3 - 1);
+    if (!t.ok) {
+      if (lane == 0) atomicOr(s.status, 4);
+      t.js = t.je = 0;
+      t.sa = t.sb = t.sc = 0;
+      t.ox = t.oy = t.oz = 0;
+    }
+    cs[q] = t.js;
+    pre[q + 1] = pre[q] + (t.je - t.js);
+    code[q] = pack_entry(0, t.sa, t.sb, t.sc);
+    ofx[q] = t.ox;
+    ofy[q] = t.oy;
+    ofz[q] = t.oz;
+  }
+  const int total = pre[9];
+  const double cut2 = s.cut2;
+  int cnt = 0;
+  for (int base = 0; base < total; base += kWave) {
+    const int t = base + lane;
+    bool keep = false;
+    double4 pj = make_double4(0, 0, 0, 0);
+    int entry = 0;
+    if (t < total) {
+      int q = 0;
+#pragma unroll
+      for (int u = 1; u < 9; ++u) q += (t >= pre[u]) ? 1 : 0;
+      int j = cs[q] + (t - pre[q]);
+      double4 p = s.spos[j];
+      pj = make_double4(p.x - ofx[q], p.y - ofy[q], p.z - ofz[q], 0.0);
+      entry = (j << kShiftBits) | code[q];
+      double dx = fmax(0.0, fmax(bl[0] - pj.x, pj.x - bh[0]));
+      double dy = fmax(0.0, fmax(bl[1] - pj.y, pj.y - bh[1]));
+      double dz = fmax(0.0, fmax(bl[2] - pj.z, pj.z - bh[2]));
+      keep = dx * dx + dy * dy + dz * dz < cut2;  // superset of every lane's acceptance
+    }
+    const unsigned long long bal = __ballot(keep);
+    const int kc = __popcll(bal);
+    if (kc == 0) continue;
+    if (keep) {
+      int slot = __popcll(bal & ((1ull << lane) - 1ull));
+      stage[wave][slot] = pj;
+      stage_e[wave][slot] = entry;
+    }
+    wave_sync();
+    unsigned long long acc = 0;
+#pragma unroll 8
+    for (int u = 0; u < kc; ++u) {
+      const double4 q = stage[wave][u];
+      const double rx = q.x - pi.x;
+      const double ry = q.y - pi.y;
+      const double rz = q.z - pi.z;
+      const double d2 = rx * rx + ry * ry + rz * rz;
+      const bool ok = (cut2 - d2 > 0.0001) & (d2 > 0.0001);
+      acc |= (unsigned long long)ok << u;
+    }
+    if (!active) acc = 0;
+    while (acc) {
+      const int u = __builtin_ctzll(acc);
+      acc &= acc - 1;
+      if (cnt < s.seg_cap)
+        s.nbr[nbr_index(seg0 + cnt, i, s.n)] = stage_e[wave][u];
       else
         atomicOr(s.status, 4);
-      L.core++;
+      cnt++;
+    }
+    wave_sync();
+  }
+  if (active) s.nbr_num[wave * s.n + i] = cnt < s.seg_cap ? cnt : s.seg_cap;
+}
+
+// Atoms whose current cell differs from the binned one (after SHAKE moves an
+// atom across a cell face, or at x == hi): the stencil is centred on
+// InWhichCell(p_i) as in the reference, so such an atom cannot share the
+// work item of its binned cell.  One wavefront per misfit atom: lanes test 64
+// candidates at a time and the accepted ones are written at their ballot
+// prefix position, which keeps the reference's traversal order.
+__global__ __launch_bounds__(kBlock) void k_nbr_misfit_verlet(SysView s) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int nw = gridDim.x * blockDim.x / kWave;
+  const int count = s.misfit_count[0];
+  for (int m = gw; m < count; m += nw) {
+    const int i = __builtin_amdgcn_readfirstlane(s.misfit_list[m]);
+    const double4 pi = s.spos[i];
+    const int ca = cell_axis(pi.x, s.lo[0], s.dxdydz[0]);
+    const int cb = cell_axis(pi.y, s.lo[1], s.dxdydz[1]);
+    const int cc = cell_axis(pi.z, s.lo[2], s.dxdydz[2]);
+    for (int a = -1; a <= 1; ++a) {
+      const int seg0 = (a + 1) * s.seg_cap;
+      int cnt = 0;
+      for (int b = -1; b <= 1; ++b)
+        for (int e = -1; e <= 1; ++e) {
+          StencilCell t = stencil_cell(s, ca, cb, cc, a, b, e);
+          if (!t.ok) {
+            if (lane == 0) atomicOr(s.status, 1);
+            continue;
+          }
+          const int code = pack_entry(0, t.sa, t.sb, t.sc);
+          for (int base = t.js; base < t.je; base += kWave) {
+            const int j = base + lane;
+            bool ok = false;
+            if (j < t.je) {
+              double4 pj = s.spos[j];
+              double rx = (pj.x - t.ox) - pi.x;
+              double ry = (pj.y - t.oy) - pi.y;
+              double rz = (pj.z - t.oz) - pi.z;
+              double d2 = rx * rx + ry * ry + rz * rz;
+              ok = s.cut2 - d2 > 0.0001 && d2 > 0.0001;
+            }
+            const unsigned long long bal = __ballot(ok);
+            if (ok) {
+              int slot = cnt + __popcll(bal & ((1ull << lane) - 1ull));
+              if (slot < s.seg_cap)
+                s.nbr[nbr_index(seg0 + slot, i, s.n)] = (j << kShiftBits) | code;
+              else
+                atomicOr(s.status, 4);
+            }
+            cnt += __popcll(bal);
+          }
+        }
+      if (lane == 0) s.nbr_num[(a + 1) * s.n + i] = cnt < s.seg_cap ? cnt : s.seg_cap;
     }
   }
 }
 
-template <bool RBL>
-__device__ __forceinline__ void finish(const SysView& s, const ListState& L, int i) {
-  if (!RBL) {
-    s.nbr_num[i] = L.cnt < s.cap ? L.cnt : s.cap;
-  } else {
-    s.nbr_num[2 * i] = L.core < s.rs_num ? L.core : s.rs_num;
-    int kept = L.shell_kept;
-    if (s.rs_num + kept > s.cap) kept = s.cap - s.rs_num;
-    s.nbr_num[2 * i + 1] = kept;
+// RBL list state: core slots [0, rs_num), kept shell slots from rs_num (every
+// pice_num-th shell atom in traversal order, so one lane walks the whole stencil)
+struct RblState {
+  int core, shell_seen, shell_kept;
+};
+
+__device__ __forceinline__ void rbl_consider(const SysView& s, RblState& L, int i, int entry, double d2) {
+  if (s.cut2 - d2 > 0.0001 && d2 - s.rs2 > 0.0001) {
+    if (L.shell_seen % s.pice_num == 0) {
+      int slot = s.rs_num + L.shell_kept;
+      if (slot < s.cap)
+        s.nbr[nbr_index(slot, i, s.n)] = entry;
+      else
+        atomicOr(s.status, 4);
+      L.shell_kept++;
+    }
+    L.shell_seen++;
+  } else if (s.rs2 - d2 > 0.0001 && d2 > 0.0001) {
+    if (L.core < s.rs_num)
+      s.nbr[nbr_index(L.core, i, s.n)] = entry;
+    else
+      atomicOr(s.status, 4);
+    L.core++;
   }
 }
 
-// one wavefront per work item (64 i-atoms of one cell), 4 items per workgroup
-template <bool RBL>
-__global__ __launch_bounds__(kBlock) void k_nbr_build(SysView s) {
+__device__ __forceinline__ void rbl_finish(const SysView& s, const RblState& L, int i) {
+  s.nbr_num[2 * i] = L.core < s.rs_num ? L.core : s.rs_num;
+  int kept = L.shell_kept;
+  if (s.rs_num + kept > s.cap) kept = s.cap - s.rs_num;
+  s.nbr_num[2 * i + 1] = kept;
+}
+
+// RBL list build: one wavefront per work item; the j loop is wave-uniform
+// (scalar loads broadcast to all lanes)
+__global__ __launch_bounds__(kBlock) void k_nbr_build_rbl(SysView s) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
   const int item_idx = blockIdx.x * (kBlock / kWave) + wave;
@@ -265,101 +492,127 @@ __global__ __launch_bounds__(kBlock) void k_nbr_build(SysView s) {
   const int c = item >> 8;
   const int chunk = item & 255;
   const int start = s.cell_start[c];
-  const int cnt = s.cell_start[c + 1] - start;
+  const int cnt_c = s.cell_start[c + 1] - start;
   const int local = chunk * kChunk + lane;
   const int i = start + local;
-  bool active = local < cnt;
+  bool active = local < cnt_c;
   double4 pi = make_double4(0, 0, 0, 0);
   if (active) {
     pi = s.spos[i];
-    if (s.misfit[i]) active = false;  // handled by k_nbr_misfit
+    if (s.misfit[i]) active = false;
   }
   const int ca = c % s.dims[0];
   const int cb = (c / s.dims[0]) % s.dims[1];
   const int cc = c / (s.dims[0] * s.dims[1]);
-  ListState L = {0, 0, 0, 0};
-  const int nc = s.nc;
-  for (int a = -nc; a <= nc; ++a) {
-    for (int b = -nc; b <= nc; ++b) {
-      for (int e = -nc; e <= nc; ++e) {
-        int na = ca + a, nb = cb + b, ne = cc + e;
-        int wa = wrap_axis(na, s.dims[0]), wb = wrap_axis(nb, s.dims[1]), we = wrap_axis(ne, s.dims[2]);
-        if (wa < 0 || wb < 0 || we < 0) {  // stencil wider than two boxes: unsupported geometry
-          if (lane == 0) atomicOr(s.status, 4);
-          continue;
-        }
-        double ox = s.dxdydz[0] * (double)(wa - na);
-        double oy = s.dxdydz[1] * (double)(wb - nb);
-        double oz = s.dxdydz[2] * (double)(we - ne);
-        int sa = (wa - na) / s.dims[0], sb = (wb - nb) / s.dims[1], sc = (we - ne) / s.dims[2];
-        int wcell = wa + wb * s.dims[0] + we * s.dims[0] * s.dims[1];
-        int js = s.cell_start[wcell], je = s.cell_start[wcell + 1];
-        for (int j = js; j < je; ++j) {
-          double4 pj = s.spos[j];  // wave-uniform address -> scalar load
-          double rx = (pj.x - ox) - pi.x;
-          double ry = (pj.y - oy) - pi.y;
-          double rz = (pj.z - oz) - pi.z;
-          double d2 = rx * rx + ry * ry + rz * rz;
-          if (active) consider<RBL>(s, L, i, pack_entry(j, sa, sb, sc), d2);
-        }
-      }
-    }
-  }
-  if (local < cnt && active) finish<RBL>(s, L, i);
-}
-
-// atoms whose current cell differs from the binned one: own stencil, one thread each
-template <bool RBL>
-__global__ __launch_bounds__(kBlock) void k_nbr_misfit(SysView s) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n || !s.misfit[i]) return;
-  double4 pi = s.spos[i];
-  int ca = cell_axis(pi.x, s.lo[0], s.dxdydz[0]);
-  int cb = cell_axis(pi.y, s.lo[1], s.dxdydz[1]);
-  int cc = cell_axis(pi.z, s.lo[2], s.dxdydz[2]);
-  ListState L = {0, 0, 0, 0};
+  RblState L = {0, 0, 0};
   const int nc = s.nc;
   for (int a = -nc; a <= nc; ++a)
     for (int b = -nc; b <= nc; ++b)
       for (int e = -nc; e <= nc; ++e) {
-        int na = ca + a, nb = cb + b, ne = cc + e;
-        int wa = wrap_axis(na, s.dims[0]), wb = wrap_axis(nb, s.dims[1]), we = wrap_axis(ne, s.dims[2]);
-        if (wa < 0 || wb < 0 || we < 0) {  // far outside the box: PeriodicIndexOffset assumes >= -dims
-          atomicOr(s.status, 1);
+        StencilCell t = stencil_cell(s, ca, cb, cc, a, b, e);
+        if (!t.ok) {
+          if (lane == 0) atomicOr(s.status, 4);
           continue;
         }
-        double ox = s.dxdydz[0] * (double)(wa - na);
-        double oy = s.dxdydz[1] * (double)(wb - nb);
-        double oz = s.dxdydz[2] * (double)(we - ne);
-        int sa = (wa - na) / s.dims[0], sb = (wb - nb) / s.dims[1], sc = (we - ne) / s.dims[2];
-        if (sa < -3 || sa > 3 || sb < -3 || sb > 3 || sc < -3 || sc > 3) {
-          atomicOr(s.status, 1);
-          continue;
-        }
-        int wcell = wa + wb * s.dims[0] + we * s.dims[0] * s.dims[1];
-        int js = s.cell_start[wcell], je = s.cell_start[wcell + 1];
-        for (int j = js; j < je; ++j) {
-          double4 pj = s.spos[j];
-          double rx = (pj.x - ox) - pi.x;
-          double ry = (pj.y - oy) - pi.y;
-          double rz = (pj.z - oz) - pi.z;
+        const int code = pack_entry(0, t.sa, t.sb, t.sc);
+        for (int j = t.js; j < t.je; ++j) {
+          double4 pj = s.spos[j];  // wave-uniform address -> scalar load
+          double rx = (pj.x - t.ox) - pi.x;
+          double ry = (pj.y - t.oy) - pi.y;
+          double rz = (pj.z - t.oz) - pi.z;
           double d2 = rx * rx + ry * ry + rz * rz;
-          consider<RBL>(s, L, i, pack_entry(j, sa, sb, sc), d2);
+          if (active) rbl_consider(s, L, i, (j << kShiftBits) | code, d2);
         }
       }
-  finish<RBL>(s, L, i);
+  if (active) rbl_finish(s, L, i);
+}
+
+// RBL misfit atoms, one wavefront each: core entries at their ballot prefix;
+// shell candidates get their ordinal in traversal order from the ballot prefix
+// and every pice_num-th one is kept, again at its ballot prefix among the kept
+__global__ __launch_bounds__(kBlock) void k_nbr_misfit_rbl(SysView s) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int nw = gridDim.x * blockDim.x / kWave;
+  const int count = s.misfit_count[0];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int m = gw; m < count; m += nw) {
+    const int i = __builtin_amdgcn_readfirstlane(s.misfit_list[m]);
+    const double4 pi = s.spos[i];
+    const int ca = cell_axis(pi.x, s.lo[0], s.dxdydz[0]);
+    const int cb = cell_axis(pi.y, s.lo[1], s.dxdydz[1]);
+    const int cc = cell_axis(pi.z, s.lo[2], s.dxdydz[2]);
+    int core = 0, seen = 0, kept = 0;
+    const int nc = s.nc;
+    for (int a = -nc; a <= nc; ++a)
+      for (int b = -nc; b <= nc; ++b)
+        for (int e = -nc; e <= nc; ++e) {
+          StencilCell t = stencil_cell(s, ca, cb, cc, a, b, e);
+          if (!t.ok) {
+            if (lane == 0) atomicOr(s.status, 1);
+            continue;
+          }
+          const int code = pack_entry(0, t.sa, t.sb, t.sc);
+          for (int base = t.js; base < t.je; base += kWave) {
+            const int j = base + lane;
+            bool is_shell = false, is_core = false;
+            if (j < t.je) {
+              double4 pj = s.spos[j];
+              double rx = (pj.x - t.ox) - pi.x;
+              double ry = (pj.y - t.oy) - pi.y;
+              double rz = (pj.z - t.oz) - pi.z;
+              double d2 = rx * rx + ry * ry + rz * rz;
+              is_shell = s.cut2 - d2 > 0.0001 && d2 - s.rs2 > 0.0001;
+              is_core = !is_shell && s.rs2 - d2 > 0.0001 && d2 > 0.0001;
+            }
+            const int entry = (j << kShiftBits) | code;
+            const unsigned long long bc = __ballot(is_core);
+            if (is_core) {
+              int slot = core + __popcll(bc & below);
+              if (slot < s.rs_num)
+                s.nbr[nbr_index(slot, i, s.n)] = entry;
+              else
+                atomicOr(s.status, 4);
+            }
+            core += __popcll(bc);
+            const unsigned long long bs = __ballot(is_shell);
+            const bool keep = is_shell && ((seen + __popcll(bs & below)) % s.pice_num == 0);
+            const unsigned long long bk = __ballot(keep);
+            if (keep) {
+              int slot = s.rs_num + kept + __popcll(bk & below);
+              if (slot < s.cap)
+                s.nbr[nbr_index(slot, i, s.n)] = entry;
+              else
+                atomicOr(s.status, 4);
+            }
+            seen += __popcll(bs);
+            kept += __popcll(bk);
+          }
+        }
+    if (lane == 0) {
+      RblState L = {core, seen, kept};
+      rbl_finish(s, L, i);
+    }
+  }
 }
 
 void launch_nbr_build(const SysView& s, hipStream_t st) {
-  int grid = (s.n_items + (kBlock / kWave) - 1) / (kBlock / kWave);
-  int gm = (s.n + kBlock - 1) / kBlock;
   if (s.rbl) {
-    hipLaunchKernelGGL(k_nbr_build<true>, dim3(grid), dim3(kBlock), 0, st, s);
-    hipLaunchKernelGGL(k_nbr_misfit<true>, dim3(gm), dim3(kBlock), 0, st, s);
+    int grid = (s.n_items + (kBlock / kWave) - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL(k_nbr_build_rbl, dim3(grid), dim3(kBlock), 0, st, s);
   } else {
-    hipLaunchKernelGGL(k_nbr_build<false>, dim3(grid), dim3(kBlock), 0, st, s);
-    hipLaunchKernelGGL(k_nbr_misfit<false>, dim3(gm), dim3(kBlock), 0, st, s);
+    hipLaunchKernelGGL(k_nbr_build_verlet, dim3(s.n_items), dim3(3 * kWave), 0, st, s);
   }
+}
+
+// grid-stride over the misfit list: up to 1024 wavefronts
+void launch_nbr_misfit(const SysView& s, hipStream_t st) {
+  int waves = s.n < 1024 ? s.n : 1024;
+  int grid = (waves * kWave + kBlock - 1) / kBlock;
+  if (s.rbl)
+    hipLaunchKernelGGL(k_nbr_misfit_rbl, dim3(grid), dim3(kBlock), 0, st, s);
+  else
+    hipLaunchKernelGGL(k_nbr_misfit_verlet, dim3(grid), dim3(kBlock), 0, st, s);
 }
 
 }  // namespace rbmd

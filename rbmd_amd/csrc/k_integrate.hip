@@ -14,69 +14,64 @@
 
 namespace rbmd {
 
-__device__ __forceinline__ int bin_axis_i(double x, double lo, double d, int dims) {
-  double t = (x - lo) / d;
-  int k = 0;
-  if (t == t) k = (int)fmax(fmin(t, 2.0e9), -2.0e9);
-  k = k < 0 ? 0 : k;
-  k = k > dims - 1 ? dims - 1 : k;
-  return k;
-}
-
 // NHC scale (optional), kick, drift, PBC, binning
 __global__ __launch_bounds__(kBlock) void k_first_half(SysView s, IntView iv, const double* scale, int do_bin) {
+  extern __shared__ int lds_cells[];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n) return;
-  double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
-  if (scale) {
-    double f = *scale;
-    vx = f * vx;
-    vy = f * vy;
-    vz = f * vz;
-  }
-  s.ovx[i] = vx;
-  s.ovy[i] = vy;
-  s.ovz[i] = vz;
-  double m = s.mass[i];
-  vx += 0.5 * s.fx[i] / m * iv.dt * iv.fmt2v;
-  vy += 0.5 * s.fy[i] / m * iv.dt * iv.fmt2v;
-  vz += 0.5 * s.fz[i] / m * iv.dt * iv.fmt2v;
-  s.vx[i] = vx;
-  s.vy[i] = vy;
-  s.vz[i] = vz;
-  double p[3] = {s.x[i], s.y[i], s.z[i]};
-  s.ox[i] = p[0];
-  s.oy[i] = p[1];
-  s.oz[i] = p[2];
-  p[0] += vx * iv.dt;
-  p[1] += vy * iv.dt;
-  p[2] += vz * iv.dt;
+  int cell = -1;
+  if (i < s.n) {
+    double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+    if (scale) {
+      double f = *scale;
+      vx = f * vx;
+      vy = f * vy;
+      vz = f * vz;
+    }
+    s.ovx[i] = vx;
+    s.ovy[i] = vy;
+    s.ovz[i] = vz;
+    double m = s.mass[i];
+    vx += 0.5 * s.fx[i] / m * iv.dt * iv.fmt2v;
+    vy += 0.5 * s.fy[i] / m * iv.dt * iv.fmt2v;
+    vz += 0.5 * s.fz[i] / m * iv.dt * iv.fmt2v;
+    s.vx[i] = vx;
+    s.vy[i] = vy;
+    s.vz[i] = vz;
+    double p[3] = {s.x[i], s.y[i], s.z[i]};
+    s.ox[i] = p[0];
+    s.oy[i] = p[1];
+    s.oz[i] = p[2];
+    p[0] += vx * iv.dt;
+    p[1] += vy * iv.dt;
+    p[2] += vz * iv.dt;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    if (p[d] < s.lo[d]) {
-      p[d] += s.box[d];
-      s.img[3 * i + d] -= 1;
-    } else if (p[d] >= s.hi[d]) {
-      p[d] -= s.box[d];
-      s.img[3 * i + d] += 1;
+    for (int d = 0; d < 3; ++d) {
+      if (p[d] < s.lo[d]) {
+        p[d] += s.box[d];
+        s.img[3 * i + d] -= 1;
+      } else if (p[d] >= s.hi[d]) {
+        p[d] -= s.box[d];
+        s.img[3 * i + d] += 1;
+      }
+    }
+    s.x[i] = p[0];
+    s.y[i] = p[1];
+    s.z[i] = p[2];
+    if (!isfinite(p[0] + p[1] + p[2] + vx + vy + vz)) atomicOr(s.status, 1);
+    if (do_bin) {
+      int a = bin_axis(p[0], s.lo[0], s.dxdydz[0], s.dims[0]);
+      int b = bin_axis(p[1], s.lo[1], s.dxdydz[1], s.dims[1]);
+      int c = bin_axis(p[2], s.lo[2], s.dxdydz[2], s.dims[2]);
+      cell = a + b * s.dims[0] + c * s.dims[0] * s.dims[1];
+      s.cell_of[i] = cell;
     }
   }
-  s.x[i] = p[0];
-  s.y[i] = p[1];
-  s.z[i] = p[2];
-  if (!isfinite(p[0] + p[1] + p[2] + vx + vy + vz)) atomicOr(s.status, 1);
-  if (do_bin) {
-    int a = bin_axis_i(p[0], s.lo[0], s.dxdydz[0], s.dims[0]);
-    int b = bin_axis_i(p[1], s.lo[1], s.dxdydz[1], s.dims[1]);
-    int c = bin_axis_i(p[2], s.lo[2], s.dxdydz[2], s.dims[2]);
-    int cell = a + b * s.dims[0] + c * s.dims[0] * s.dims[1];
-    s.cell_of[i] = cell;
-    atomicAdd(&s.cell_count[cell], 1);
-  }
+  if (do_bin) count_cells(s, cell, lds_cells);
 }
 
 void launch_first_half(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin) {
-  hipLaunchKernelGGL(k_first_half, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, iv, scale, do_bin);
+  size_t lds = (do_bin && s.lds_bins) ? (size_t)s.ncell * sizeof(int) : 0;
+  hipLaunchKernelGGL(k_first_half, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), lds, st, s, iv, scale, do_bin);
 }
 
 struct D3 {

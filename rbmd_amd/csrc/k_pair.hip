@@ -58,10 +58,72 @@ __device__ __forceinline__ void vir_add(double* v, double rx, double ry, double 
   v[5] += ry * fz;
 }
 
-// MODE 0: plain LJ+coul, 1: weighted (special bonds), 2: LJ only
+template <int W>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = 1; off < W; off <<= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// One pair term; returns the scalar f such that the force on i is f * r.
+// Reference arithmetic: LJ ExecForceFunction.h:412-435 / 818-841, G(r) :682-689,
+// special weights RunWorklet.cpp:2827-2857.
+template <int MODE, bool EN>
+__device__ __forceinline__ double pair_term(const SysView& s, double rx, double ry, double rz, double d2, double qq,
+                                            double epsij, double sig6, double w, double& elj, double& ec) {
+  const bool in = d2 < s.cut2 && d2 > 0.0001;
+  double flj = 0, fc = 0;
+  elj = 0;
+  ec = 0;
+  if (in) {
+    double d6 = d2 * d2 * d2;
+    double sd6 = sig6 / d6;
+    flj = -24.0 * epsij * ((2.0 * sd6 - 1.0) * sd6) / d2;
+    if (EN) elj = 0.5 * (4.0 * epsij * (sd6 - 1.0) * sd6);
+    if (MODE != 2) {
+      double rinv = rsqrt(d2);
+      double dis = d2 * rinv;
+      double erfcv = erfc(s.sqrt_alpha * dis);
+      double g = erfcv * (rinv * rinv) + (2.0 / sqrt(kPi)) * s.sqrt_alpha * exp(-s.alpha * d2) * rinv;
+      if (MODE == 0) {
+        fc = s.qqr2e * (-qq * g * rinv);
+        if (EN) ec = s.qqr2e * (0.5 * qq * erfcv * rinv);
+      } else {
+        fc = s.qqr2e * (-qq * g * rinv + (1.0 - w) * qq * (rinv * rinv * rinv));
+        if (EN) ec = s.qqr2e * (0.5 * qq * erfcv * rinv - (1.0 - w) * (0.5 * qq * rinv));
+      }
+    }
+  }
+  if (MODE == 1) {
+    flj *= w;
+    elj *= w;
+  }
+  return fc + flj;
+}
+
+constexpr int kPairBatch = 4;
+constexpr int kMaxTypes = 16;
+
+// MODE 0: plain LJ+coul, 1: weighted (special bonds), 2: LJ only.
+// kLanes consecutive threads share one atom: lane `sub` takes list slots
+// sub, sub + kLanes, ... of every segment, kPairBatch slots at a time so that
+// several neighbour gathers are in flight per lane, and the partial sums are
+// combined with a fixed xor-shuffle tree at the end (deterministic).
 template <int MODE, bool THERMO, bool PERATOM>
 __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double tab_eps[kMaxTypes * kMaxTypes], tab_sig[kMaxTypes * kMaxTypes];
+  const int nt = s.ntypes;
+  const bool lds_tab = nt <= kMaxTypes;
+  if (lds_tab)
+    for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
+      tab_eps[k] = s.eps_ij[k];
+      tab_sig[k] = s.sig6_ij[k];
+    }
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / kLanes;
+  const int sub = t % kLanes;
+  constexpr bool EN = THERMO || PERATOM;
   double fx = 0, fy = 0, fz = 0;
   double vir[6] = {0, 0, 0, 0, 0, 0};
   double pelj = 0, pec = 0;
@@ -86,74 +148,74 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
         spw[k] = k < spn ? s.sp_w[b + k] : 1.0;
       }
     }
-    const int num = s.nbr_num[i];
-    const double* eps_row = s.eps_ij + ti * s.ntypes;
-    const double* sig_row = s.sig6_ij + ti * s.ntypes;
-    for (int k = 0; k < num; ++k) {
-      int j;
-      double ox, oy, oz;
-      decode(s, s.nbr[(size_t)k * s.n + i], j, ox, oy, oz);
-      const double4 pj = s.spos[j];
-      double rx = (pj.x - ox) - pi.x;
-      double ry = (pj.y - oy) - pi.y;
-      double rz = (pj.z - oz) - pi.z;
-      min_image(s, rx, ry, rz);
-      const double d2 = rx * rx + ry * ry + rz * rz;
-      const int tj = s.stype[j];
-      const double epsij = eps_row[tj];
-      const double sig6 = sig_row[tj];
-      const bool in = d2 < s.cut2 && d2 > 0.0001;
-      double flj = 0, elj = 0;
-      if (in) {
-        double d6 = d2 * d2 * d2;
-        double sd6 = sig6 / d6;
-        flj = -24.0 * epsij * ((2.0 * sd6 - 1.0) * sd6) / d2;
-        if (THERMO || PERATOM) elj = 0.5 * (4.0 * epsij * (sd6 - 1.0) * sd6);
-      }
-      if (MODE == 2) {
-        double px = flj * rx, py = flj * ry, pz = flj * rz;
-        fx += px;
-        fy += py;
-        fz += pz;
-        if (THERMO || PERATOM) {
-          vir_add(vir, rx, ry, rz, px, py, pz, 0.5);
-          pelj += elj;
-        }
-        continue;
-      }
-      const double qj = pj.w;
-      double w = 1.0;
-      if (MODE == 1) {
+    for (int seg = 0; seg < s.nseg; ++seg) {
+      const int num = s.nbr_num[seg * s.n + i];
+      const int base = seg * s.seg_cap;
+      for (int k0 = sub; k0 < num; k0 += kLanes * kPairBatch) {
+        int ent[kPairBatch];
 #pragma unroll
-        for (int t = 0; t < kMaxSpecial; ++t)
-          if (spj[t] == j) w = spw[t];
-      }
-      double fc = 0, ec = 0;
-      if (in) {
-        double dis = sqrt(d2);
-        double erfcv = erfc(s.sqrt_alpha * dis);
-        double g = erfcv / (dis * dis) + 2.0 * s.sqrt_alpha * exp(-s.alpha * d2) / (sqrt(kPi) * dis);
-        double qq = qi * qj;
-        if (MODE == 0) {
-          fc = s.qqr2e * (-qq * g / dis);
-          if (THERMO || PERATOM) ec = s.qqr2e * (0.5 * qq * erfcv / dis);
-        } else {
-          fc = s.qqr2e * (-qq * g / dis - (1.0 - w) * (-qq / (d2 * dis)));
-          if (THERMO || PERATOM) ec = s.qqr2e * (0.5 * qq * erfcv / dis - (1.0 - w) * (0.5 * qq / dis));
+        for (int bb = 0; bb < kPairBatch; ++bb) {
+          int k = k0 + bb * kLanes;
+          ent[bb] = k < num ? s.nbr[nbr_index(base + k, i, s.n)] : -1;
         }
-      }
-      double lw = (MODE == 1) ? w : 1.0;
-      double sc = fc + lw * flj;
-      double px = sc * rx, py = sc * ry, pz = sc * rz;
-      fx += px;
-      fy += py;
-      fz += pz;
-      if (THERMO || PERATOM) {
-        vir_add(vir, rx, ry, rz, px, py, pz, MODE == 0 ? 0.5 : -0.5);
-        pelj += lw * elj;
-        pec += ec;
+        double4 pj[kPairBatch];
+        int tj[kPairBatch];
+#pragma unroll
+        for (int bb = 0; bb < kPairBatch; ++bb) {
+          int j = ent[bb] >= 0 ? (ent[bb] >> kShiftBits) : i;
+          pj[bb] = s.spos[j];
+          tj[bb] = s.stype[j];
+        }
+#pragma unroll
+        for (int bb = 0; bb < kPairBatch; ++bb) {
+          if (ent[bb] < 0) continue;
+          int j;
+          double ox, oy, oz;
+          decode(s, ent[bb], j, ox, oy, oz);
+          double rx = (pj[bb].x - ox) - pi.x;
+          double ry = (pj[bb].y - oy) - pi.y;
+          double rz = (pj[bb].z - oz) - pi.z;
+          min_image(s, rx, ry, rz);
+          const double d2 = rx * rx + ry * ry + rz * rz;
+          const int tix = ti * nt + tj[bb];
+          const double epsij = lds_tab ? tab_eps[tix] : s.eps_ij[tix];
+          const double sig6 = lds_tab ? tab_sig[tix] : s.sig6_ij[tix];
+          double w = 1.0;
+          if (MODE == 1) {
+#pragma unroll
+            for (int u = 0; u < kMaxSpecial; ++u)
+              if (spj[u] == j) w = spw[u];
+          }
+          double elj, ec;
+          const double sc = pair_term<MODE, EN>(s, rx, ry, rz, d2, qi * pj[bb].w, epsij, sig6, w, elj, ec);
+          const double px = sc * rx, py = sc * ry, pz = sc * rz;
+          fx += px;
+          fy += py;
+          fz += pz;
+          if (EN) {
+            vir_add(vir, rx, ry, rz, px, py, pz, MODE == 1 ? -0.5 : 0.5);
+            pelj += elj;
+            pec += ec;
+          }
+        }
       }
     }
+  }
+  if (THERMO) {
+    const double vals[8] = {pelj, pec, vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
+    const int slots[8] = {S_ELJ, S_ECOUL, S_VIR_NEAR, S_VIR_NEAR + 1, S_VIR_NEAR + 2,
+                          S_VIR_NEAR + 3, S_VIR_NEAR + 4, S_VIR_NEAR + 5};
+    block_write_row<8>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+  }
+  fx = group_sum<kLanes>(fx);
+  fy = group_sum<kLanes>(fy);
+  fz = group_sum<kLanes>(fz);
+  if (PERATOM) {
+    for (int k = 0; k < 6; ++k) vir[k] = group_sum<kLanes>(vir[k]);
+    pelj = group_sum<kLanes>(pelj);
+    pec = group_sum<kLanes>(pec);
+  }
+  if (i < s.n && sub == 0) {
     s.fnx[i] = fx;
     s.fny[i] = fy;
     s.fnz[i] = fz;
@@ -162,12 +224,6 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
       s.e_lj[i] = pelj;
       s.e_coul[i] = pec;
     }
-  }
-  if (THERMO) {
-    const double vals[8] = {pelj, pec, vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
-    const int slots[8] = {S_ELJ, S_ECOUL, S_VIR_NEAR, S_VIR_NEAR + 1, S_VIR_NEAR + 2,
-                          S_VIR_NEAR + 3, S_VIR_NEAR + 4, S_VIR_NEAR + 5};
-    block_write_row<8>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
   }
 }
 
@@ -208,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_rbl(SysView s) {
     for (int k = 0; k < num; ++k) {
       int j;
       double ox, oy, oz;
-      decode(s, s.nbr[(size_t)(base + k) * s.n + i], j, ox, oy, oz);
+      decode(s, s.nbr[nbr_index(base + k, i, s.n)], j, ox, oy, oz);
       const double4 pj = s.spos[j];
       double rx = (pj.x - ox) - pi.x;
       double ry = (pj.y - oy) - pi.y;
@@ -306,9 +362,12 @@ __global__ __launch_bounds__(1024) void k_rbl_mean(SysView s) {
 }
 
 void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int* nblocks) {
-  int grid = (s.n + kBlock - 1) / kBlock;
+  int grid_rbl = (s.n + kBlock - 1) / kBlock;
+  int grid = (s.n * kLanes + kBlock - 1) / kBlock;
   if (nblocks) *nblocks = grid;
   if (s.rbl && !s.lj_only) {
+    grid = grid_rbl;
+    if (nblocks) *nblocks = grid;
     if (s.weighted)
       hipLaunchKernelGGL(k_pair_rbl<true>, dim3(grid), dim3(kBlock), 0, st, s);
     else

@@ -36,4 +36,34 @@ __device__ __forceinline__ void block_write_row(const double (&vals)[NV], const 
   }
 }
 
+// BinPointsWorklet (ContPointLocator.cxx:38-66): ijk = trunc((x - min) / dxdydz),
+// clamped to [0, dims-1]; a NaN coordinate lands in cell 0 (and is flagged elsewhere)
+__device__ __forceinline__ int bin_axis(double x, double lo, double d, int dims) {
+  double t = (x - lo) / d;
+  int k = 0;
+  if (t == t) k = (int)fmax(fmin(t, 2.0e9), -2.0e9);  // trunc toward zero like the Vec3f->Id3 cast
+  k = k < 0 ? 0 : k;
+  k = k > dims - 1 ? dims - 1 : k;
+  return k;
+}
+
+// Count one atom per thread into cell_count.  With few cells the workgroup
+// first aggregates in LDS, so each cell takes one global atomic per workgroup
+// instead of one per atom.  Every thread of the workgroup must call this
+// (cell < 0: nothing to count).
+__device__ __forceinline__ void count_cells(const SysView& s, int cell, int* lds) {
+  if (s.lds_bins) {
+    for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) lds[c] = 0;
+    __syncthreads();
+    if (cell >= 0) atomicAdd(&lds[cell], 1);
+    __syncthreads();
+    for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) {
+      int v = lds[c];
+      if (v) atomicAdd(&s.cell_count[c], v);
+    }
+  } else if (cell >= 0) {
+    atomicAdd(&s.cell_count[cell], 1);
+  }
+}
+
 }  // namespace rbmd
