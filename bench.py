@@ -8,9 +8,12 @@ cell list, SHAKE, neighbour lists, real-space LJ + erfc, RBE structure factor
 and force, bonded terms, kick, RATTLE) with all state resident in HBM.
 
 Prints one JSON line.  Extra objects:
-  roofline      dominant kernel: algorithmic FP64 flops per launch (43 per list
-                entry) / its mean duration measured with HIP events on the
-                library's stream, against the FP64 vector peak (78.6 TFLOP/s)
+  roofline      dominant kernel: algorithmic FP64 flops per launch / its mean
+                duration measured with HIP events on the library's stream,
+                against the FP64 vector peak (78.6 TFLOP/s).  Algorithmic work
+                is the reference algorithm's: 8 flops per distance test of an
+                atom against every atom of its 27 stencil cells, 43 per
+                accepted pair (pair term, counted from each side)
   cpu_baseline  the CPU oracle (a C++ restatement of the reference
                 algorithms) timed on this host's cores on a few steps of the
                 same workload
@@ -26,8 +29,8 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-PAIR_FLOPS = 43.0  # per list entry (pair counted from each side), see DESIGN.md
-RBE_FLOPS = 39.0 + 35.0  # per (atom, k) term: structure factor + force
+PAIR_FLOPS = 43.0  # per accepted pair (counted from each side): LJ + erfc-Coulomb force
+TEST_FLOPS = 8.0  # per distance test of an atom against a stencil candidate
 
 
 def workload(name):
@@ -48,6 +51,22 @@ def workload(name):
     else:
         raise SystemExit(f"unknown workload {name}")
     return s, r, desc, dt_fs
+
+
+def stencil_candidates(g):
+    """sum over atoms of the atom counts of their 27 stencil cells (the distance
+    tests of the reference's neighbour search)"""
+    import numpy as np
+
+    dims = g.geometry()[0]
+    _, _, lower, upper = g.locator()
+    cnt = (np.asarray(upper) - np.asarray(lower)).reshape(dims[2], dims[1], dims[0]).astype(np.int64)
+    nb = np.zeros_like(cnt)
+    for a in (-1, 0, 1):
+        for b in (-1, 0, 1):
+            for c in (-1, 0, 1):
+                nb += np.roll(cnt, (c, b, a), axis=(0, 1, 2))
+    return int((cnt * nb).sum())
 
 
 def cpu_baseline(system, run, steps, threads):
@@ -101,12 +120,16 @@ def main():
     g.md_step(args.profile_steps)
     rec = ctx.profile_records()
     ctx.profile(False)
+    cands = stencil_candidates(g)
+    g.build_neighbors(0)
     entries = g.neighbor_entries()
     kernels = {k: {"launches": n, "mean_us": 1e3 * ms / max(n, 1)} for k, (n, ms) in rec.items()}
     dom = max(rec.items(), key=lambda kv: kv[1][1])[0]
     P = run.rbe_p
     flops = {
+        "force_cells": PAIR_FLOPS * entries + TEST_FLOPS * cands,
         "pair": PAIR_FLOPS * entries,
+        "nbr_build": TEST_FLOPS * cands,
         "kforce": 35.0 * system.n_atoms * P,
         "rho_partial": 39.0 * system.n_atoms * P,
     }
@@ -135,7 +158,9 @@ def main():
         "ns_per_day": round((args.steps / elapsed) * dt_fs * 86400 / 1e6, 3) if dt_fs else None,
         "roofline": roof,
         "kernels_us": {k: round(v["mean_us"], 2) for k, v in kernels.items()},
-        "list_entries": entries,
+        "pairs": entries,
+        "stencil_candidates": cands,
+        "misfit_atoms": th["misfit_atoms"],
         "final_temperature": th["temperature"],
         "status_flags": th["status_flags"],
     }

@@ -66,6 +66,12 @@ enum { RBMD_PREC_FP64 = 0, RBMD_PREC_MIXED = 1 };
  * fixed alpha=0.1 step table of the RBL path (ExecStaticTable.h:96-135). */
 enum { RBMD_ERFC_ANALYTIC = 0, RBMD_ERFC_TABLE = 1 };
 
+/* VERLET real-space path.  CELLS: the neighbour search and the pair term in
+ * one fused pass over the cell stencil (no stored list; same pair set and
+ * pair term, force sums in a different order).  LISTS: build the Verlet lists
+ * (rbmd_neighbors_*), then evaluate the pair term over them. */
+enum { RBMD_PAIR_LISTS = 0, RBMD_PAIR_CELLS = 1 };
+
 /* Atom/topology description of one periodic system.
  * Mirrors what ModelFileInitCondition puts into Para
  * (framework/InitCondition/ModelFileInitCondition.cpp:406-478, 736-956). */
@@ -123,6 +129,7 @@ typedef struct rbmd_params {
   uint64_t seed;                  /* k-vector sampler seed */
   int32_t energy_every;           /* k-grid Ewald energy every n steps, 0 = never */
   double rbl_rho;                 /* number density used by RBL capacities; 0 = N/V */
+  int32_t pair_path;              /* VERLET real space: RBMD_PAIR_* */
 } rbmd_params;
 
 /* Thermodynamic scalars after a step; per-atom averages as in Para

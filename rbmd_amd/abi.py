@@ -27,6 +27,7 @@ SHAKE = {"null": 0, "true": 1, "false": 2}
 TEMP_CTRL = {"NOSE_HOOVER": 0, "BERENDSEN": 1, "TEMP_RESCALE": 2, "NONE": 3}
 PRECISION = {"fp64": 0, "mixed": 1}
 ERFC = {"analytic": 0, "table": 1}
+PAIR_PATH = {"lists": 0, "cells": 1}
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -82,6 +83,7 @@ class Params(C.Structure):
         ("seed", C.c_uint64),
         ("energy_every", C.c_int32),
         ("rbl_rho", C.c_double),
+        ("pair_path", C.c_int32),
     ]
 
 
@@ -198,4 +200,5 @@ def make_params(run):
     p.seed = int(run.seed) & 0xFFFFFFFFFFFFFFFF
     p.energy_every = int(run.energy_every)
     p.rbl_rho = float(run.rbl_rho)
+    p.pair_path = PAIR_PATH[run.pair_path]
     return p

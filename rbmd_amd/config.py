@@ -42,6 +42,7 @@ class RunConfig:
     seed: int = 12345
     energy_every: int = 0
     rbl_rho: float = 0.0
+    pair_path: str = "lists"
     thermo_interval: int = 0
 
 
@@ -150,6 +151,7 @@ def run_config_from_json(doc: dict) -> RunConfig:
     rc.seed = int(ext2.get("seed", rc.seed))
     rc.energy_every = int(ext2.get("energy_every", rc.thermo_interval))
     rc.rbl_rho = float(ext2.get("rbl_rho", 0.0))
+    rc.pair_path = str(ext2.get("pair_path", "lists"))
     return rc
 
 

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <thread>
@@ -134,6 +135,12 @@ struct rbmd_ctx {
   }
 };
 
+// producers of thermo partial rows (row widths kRowPair, kRowK, kRowBond, kRowKe, kRowShake)
+enum { SEG_PAIR, SEG_K, SEG_BOND, SEG_KE, SEG_SHAKE, NSEG };
+constexpr int kSegWidth[NSEG] = {kRowPair, kRowK, kRowBond, kRowKe, kRowShake};
+constexpr int kSegSlot0[NSEG] = {S_ELJ, S_VIR_K, S_VIR_BOND, S_KE, S_VIR_SHAKE};
+constexpr int kSegGroups[NSEG] = {64, 16, 16, 64, 16};  // 992 reduction threads in all
+
 struct rbmd_sys {
   rbmd_ctx* ctx = nullptr;
   rbmd_params p;
@@ -144,9 +151,14 @@ struct rbmd_sys {
   KView kv;
   std::vector<void*> allocs;
   int64_t bytes = 0;
-  // partial-sum rows: pair, kspace, bonded, ke, shake
+  // partial-sum rows, one segment per producer; seg_used = rows its last
+  // launch wrote (the reduction reads exactly those)
   double* rows = nullptr;
-  int nrows = 0, row_pair = 0, row_k = 0, row_bond = 0, row_ke = 0, row_shake = 0;
+  size_t seg_off[NSEG] = {0, 0, 0, 0, 0};
+  int seg_used[NSEG] = {0, 0, 0, 0, 0};
+  double* seg_rows(int g) { return rows + seg_off[g]; }
+  size_t rows_len = 0;
+  bool fused = false;  // VERLET real space through the fused cell-stencil kernel
   NhcState* nhc = nullptr;
   NhcParams np;
   int* shake_iter = nullptr;
@@ -241,36 +253,99 @@ void rbl_capacity(const rbmd_params& p, int n, double volume, int* rs_num, int* 
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-// Default list capacity.  VERLET: each of the 3 x-slab segments holds the mean
-// full-sphere count + 64 (a slab holds at most ~70% of the sphere);
-// RBL: the core region plus twice the expected kept shell atoms + 64.
+// Piecewise polynomial table of the Coulomb radial factors over [0, rc]:
+// interval k = [k h, (k+1) h], local variable u = r/h - k - 1/2 in [-1/2, 1/2],
+// degree-kCoulD interpolation at Chebyshev nodes, solved in long double.
+// P(r) = erfc(beta r) + 2 beta/sqrt(pi) r exp(-beta^2 r^2),  Q(r) = erfc(beta r).
+std::vector<double> coul_table(double beta, double rc) {
+  const int M = kCoulM, D = kCoulD, K = D + 1;
+  const long double h = (long double)rc / M;
+  const long double two_over_sqrtpi = 1.1283791670955125738961589031215452L;
+  std::vector<double> tab((size_t)M * 2 * K);
+  long double u[K];
+  for (int j = 0; j < K; ++j) u[j] = 0.5L * cosl(3.14159265358979323846264338327950288L * (j + 0.5L) / K);
+  for (int k = 0; k < M; ++k) {
+    for (int f = 0; f < 2; ++f) {
+      long double A[K][K + 1];
+      for (int j = 0; j < K; ++j) {
+        long double r = (k + 0.5L + u[j]) * h;
+        long double br = (long double)beta * r;
+        long double y = erfcl(br);
+        if (f == 0) y += two_over_sqrtpi * (long double)beta * r * expl(-br * br);
+        long double pw = 1;
+        for (int d = 0; d < K; ++d) {
+          A[j][d] = pw;
+          pw *= u[j];
+        }
+        A[j][K] = y;
+      }
+      for (int col = 0; col < K; ++col) {  // Gaussian elimination with partial pivoting
+        int piv = col;
+        for (int rr = col + 1; rr < K; ++rr)
+          if (fabsl(A[rr][col]) > fabsl(A[piv][col])) piv = rr;
+        for (int cc = 0; cc <= K; ++cc) std::swap(A[col][cc], A[piv][cc]);
+        for (int rr = 0; rr < K; ++rr) {
+          if (rr == col) continue;
+          long double fct = A[rr][col] / A[col][col];
+          for (int cc = col; cc <= K; ++cc) A[rr][cc] -= fct * A[col][cc];
+        }
+      }
+      for (int d = 0; d < K; ++d) tab[((size_t)k * 2 + f) * K + d] = (double)(A[d][K] / A[d][d]);
+    }
+  }
+  return tab;
+}
+
+// Largest fraction of a cutoff sphere that one stencil column (a prism of
+// cross-section wx x wy, infinite along z for this purpose) can hold: the
+// overlap of a ball with a symmetric box is largest when the ball sits at the
+// box centre.  Midpoint rule over the column cross-section.
+double column_fraction(double wx, double wy, double rc) {
+  const int m = 128;
+  double hx = 0.5 * wx / rc, hy = 0.5 * wy / rc;
+  if (hx > 1.0) hx = 1.0;
+  if (hy > 1.0) hy = 1.0;
+  double acc = 0.0;
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) {
+      double x = ((a + 0.5) / m * 2.0 - 1.0) * hx, y = ((b + 0.5) / m * 2.0 - 1.0) * hy;
+      double z2 = 1.0 - x * x - y * y;
+      if (z2 > 0) acc += 2.0 * std::sqrt(z2);
+    }
+  return acc * (2.0 * hx / m) * (2.0 * hy / m) / (4.0 / 3.0 * kPi);
+}
+
+// Default list capacity.  VERLET: per column segment, 1.15 x the mean count
+// in the largest column share of the cutoff sphere + 64; RBL: the core region
+// plus twice the mean shell sample + 64.
 int default_cap(const rbmd_sys* s) {
   double rho = s->n / s->v.volume;
   double rc = s->p.cut_off;
   double c = rho * 4.0 / 3.0 * kPi * rc * rc * rc;
   int st = (2 * s->v.nc + 1);
   long maxc = (long)std::max(s->n, 1) * st * st * st;
-  if (s->v.rbl) return round_up((int)std::min<long>(s->v.rs_num + (int)(2.0 * c / s->v.pice_num + 64), maxc), kLanes);
-  return (int)std::min<long>((long)(c + 64), maxc);
+  if (s->v.rbl) return round_up((int)std::min<long>(s->v.rs_num + (int)(2.0 * c / s->v.pice_num + 64), maxc), kMaxLanes);
+  double f = column_fraction(s->v.dxdydz[0], s->v.dxdydz[1], rc);
+  return (int)std::min<long>((long)(1.15 * f * c + 64), maxc);
 }
 
 // cap: the width the caller exports (0: default).  VERLET keeps one segment of
-// round_up(cap) slots per x-slab, so no segment can overflow below cap.
+// round_up(cap) slots per stencil column, so no segment can overflow below cap.
 rbmd_status ensure_nbr(rbmd_sys* s, int cap) {
   int seg_cap, nseg, export_cap;
   if (s->v.rbl) {
     nseg = 1;
     export_cap = cap > 0 ? cap : default_cap(s);
-    seg_cap = round_up(export_cap, kLanes);
+    seg_cap = round_up(export_cap, kMaxLanes);
     if (seg_cap < s->v.rs_num + 1) return fail(RBMD_ERR_ARG, "RBL capacity %d below the core region %d", cap, s->v.rs_num);
   } else {
-    nseg = 3;
+    nseg = 9;
     if (cap > 0) {
-      seg_cap = round_up(cap, kLanes);
+      seg_cap = round_up(cap, kMaxLanes);
       export_cap = cap;
     } else {
-      seg_cap = round_up(default_cap(s), kLanes);
-      export_cap = 3 * seg_cap;
+      seg_cap = round_up(default_cap(s), kMaxLanes);
+      export_cap = nseg * seg_cap;
     }
   }
   s->export_cap = export_cap;
@@ -311,13 +386,41 @@ void build_cells(rbmd_sys* s, bool bin_from_x) {
   c->run("cell_sort", [&] { launch_cell_sort(s->v, c->stream); });
 }
 
+// sorted copies of the new binning; the stored lists only on the list path
 void gather_and_list(rbmd_sys* s) {
   rbmd_ctx* c = s->ctx;
   c->run("gather", [&] { launch_gather(s->v, c->stream); });
-  c->run("nbr_build", [&] { launch_nbr_build(s->v, c->stream); });
-  c->run("nbr_misfit", [&] { launch_nbr_misfit(s->v, c->stream); });
   s->locator_built = true;
+  if (s->fused) {
+    s->nbr_built = false;
+    return;
+  }
+  c->run("nbr_build", [&] { launch_nbr_build(s->v, c->stream); });
   s->nbr_built = true;
+}
+
+// reduction descriptor of the thermo rows (nseg = 0: no reduction)
+RowsDesc rows_desc(rbmd_sys* s, bool reduce) {
+  RowsDesc d;
+  d.nseg = reduce ? NSEG : 0;
+  for (int g = 0; g < NSEG; ++g)
+    d.seg[g] = RowSeg{s->seg_rows(g), s->seg_used[g], kSegWidth[g], kSegSlot0[g], kSegGroups[g]};
+  return d;
+}
+
+// real-space force on the current cell list: fused kernel or list pair kernel
+void launch_real(rbmd_sys* s, int thermo, int peratom) {
+  rbmd_ctx* c = s->ctx;
+  int nblk = 0;
+  if (s->fused) {
+    s->v.nslab = 3;
+    c->run("force_cells", [&] { launch_force_cells(s->v, c->stream, thermo, peratom, s->seg_rows(SEG_PAIR), &nblk); });
+  } else {
+    s->v.nslab = 1;
+    c->run("pair", [&] { launch_pair(s->v, c->stream, thermo, peratom, s->seg_rows(SEG_PAIR), &nblk); });
+    if (s->v.rbl) nblk = 0;  // the RBL kernels write no rows
+  }
+  if (thermo || peratom) s->seg_used[SEG_PAIR] = nblk;
 }
 
 bool has_k(const rbmd_sys* s) {
@@ -344,8 +447,7 @@ rbmd_status ensure_rho(rbmd_sys* s, int P) {
 // ExecutionNVT.cpp:195-235).  kvec: device pointer to this step's k vectors.
 void compute_force(rbmd_sys* s, const double* kvec, int thermo) {
   rbmd_ctx* c = s->ctx;
-  double* rows = s->rows;
-  c->run("pair", [&] { launch_pair(s->v, c->stream, thermo, 0, rows + (size_t)s->row_pair * S_NSLOT, nullptr); });
+  launch_real(s, thermo, 0);
   if (has_k(s)) {
     KView k = s->kv;
     k.P = s->P;
@@ -354,12 +456,17 @@ void compute_force(rbmd_sys* s, const double* kvec, int thermo) {
     int ewald = s->p.coulomb_type == RBMD_COULOMB_EWALD;
     c->run("rho_partial", [&] { launch_rho_partial(s->v, k, c->stream); });
     c->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, c->stream); });
+    int nblk = 0;
     c->run("kforce", [&] {
-      launch_kforce(s->v, k, c->stream, ewald, s->p.alpha, thermo, 0, rows + (size_t)s->row_k * S_NSLOT, nullptr);
+      launch_kforce(s->v, k, c->stream, ewald, s->p.alpha, thermo, 0, s->seg_rows(SEG_K), &nblk);
     });
+    if (thermo) s->seg_used[SEG_K] = nblk;
   }
-  if (has_bonded(s))
-    c->run("bonded", [&] { launch_bonded(s->v, s->iv, c->stream, thermo, rows + (size_t)s->row_bond * S_NSLOT, nullptr); });
+  if (has_bonded(s)) {
+    int nblk = 0;
+    c->run("bonded", [&] { launch_bonded(s->v, s->iv, c->stream, thermo, s->seg_rows(SEG_BOND), &nblk); });
+    if (thermo) s->seg_used[SEG_BOND] = nblk;
+  }
 }
 
 // k vectors of one MD step (stream id = step index), staged through a
@@ -543,6 +650,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     if (d->type[i] < 0 || d->type[i] >= d->n_types) return fail(RBMD_ERR_ARG, "atom %d has type %d", i, d->type[i]);
     if (!(d->mass[i] > 0)) return fail(RBMD_ERR_ARG, "atom %d has non-positive mass", i);
   }
+  if (p->pair_path != RBMD_PAIR_CELLS && p->pair_path != RBMD_PAIR_LISTS) return fail(RBMD_ERR_ARG, "bad pair_path");
   if (p->coulomb_type == RBMD_COULOMB_RBE && p->rbe_p <= 0) return fail(RBMD_ERR_ARG, "coulomb_sample_num must be > 0");
   if (p->coulomb_type != RBMD_COULOMB_NONE && !(p->alpha > 0)) return fail(RBMD_ERR_ARG, "alpha must be > 0");
   if (d->special_offsets && d->special_ids && d->special_weights) {
@@ -581,11 +689,29 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   v.lj_only = p->force_field == RBMD_FF_LJ_CUT;
   v.rbl = p->neighbor_type == RBMD_NEIGHBOR_RBL;
   v.erfc_table = p->erfc_mode == RBMD_ERFC_TABLE;
+  // accepted pairs have |r| < rc, so the minimum image is a no-op unless an edge is below 2 rc
+  v.use_min_image = (v.box[0] < 2 * v.cut || v.box[1] < 2 * v.cut || v.box[2] < 2 * v.cut) ? 1 : 0;
+  // the fused kernel keeps the type-pair tables in LDS; more types take the list path
+  s->fused = !v.rbl && p->pair_path == RBMD_PAIR_CELLS && d->n_types <= kMaxTypes;
   bool have_sp = d->special_offsets && d->special_ids && d->special_weights;
   v.weighted = (have_sp && p->special_bonds) ? 1 : 0;
   if (v.rbl) rbl_capacity(*p, n, v.volume, &v.rs_num, &v.pice_num);
   v.n_items = v.ncell + n / kChunk + 1;
   v.lds_bins = v.ncell <= 4096 ? 1 : 0;
+  // lanes per atom in the pair kernel: enough threads to fill 256 CUs for
+  // small systems, fewer lanes (longer per-lane loops) for large ones
+  v.lanes_lg = n <= 60000 ? 4 : (n <= 250000 ? 3 : 2);
+  if (const char* e = std::getenv("RBMD_PAIR_LANES_LG")) {
+    int lg = std::atoi(e);
+    if (lg >= 2 && lg <= 4) v.lanes_lg = lg;
+  }
+  v.sp_max = 0;
+  if (d->special_offsets && d->special_ids && d->special_weights)
+    for (int i = 0; i < n; ++i) v.sp_max = std::max(v.sp_max, d->special_offsets[i + 1] - d->special_offsets[i]);
+  if (v.sp_max > 16) {
+    delete s;
+    return fail(RBMD_ERR_UNSUPPORTED, "special lists longer than 16 entries");
+  }
 
   rbmd_status rc = RBMD_OK;
   auto A = [&](rbmd_status r) {
@@ -593,8 +719,13 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   };
   // per-atom state
   double** fields[] = {&v.x, &v.y, &v.z, &v.vx, &v.vy, &v.vz, &v.fx, &v.fy, &v.fz, &v.ox, &v.oy, &v.oz,
-                       &v.ovx, &v.ovy, &v.ovz, &v.fnx, &v.fny, &v.fnz, &v.fkx, &v.fky, &v.fkz, &v.fbx, &v.fby, &v.fbz};
+                       &v.ovx, &v.ovy, &v.ovz, &v.fkx, &v.fky, &v.fkz, &v.fbx, &v.fby, &v.fbz};
   for (double** f : fields) A(s->dalloc(f, n));
+  // real-space outputs hold up to 3 slab partials (fused path)
+  double** slabbed[] = {&v.fnx, &v.fny, &v.fnz, &v.e_lj, &v.e_coul};
+  for (double** f : slabbed) A(s->dalloc(f, 3 * (size_t)n));
+  A(s->dalloc(&v.vir_near, 18 * (size_t)n));
+  v.nslab = 1;
   A(s->upload(&v.q, d->charge, n));
   A(s->upload(&v.mass, d->mass, n));
   A(s->upload(&v.type, d->type, n));
@@ -609,6 +740,13 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     }
   A(s->upload(&v.eps_ij, eij.data(), eij.size()));
   A(s->upload(&v.sig6_ij, s6.data(), s6.size()));
+  {
+    std::vector<double> ct = coul_table(v.sqrt_alpha > 0 ? v.sqrt_alpha : 0.0, p->cut_off);
+    double* dct = nullptr;
+    A(s->upload(&dct, ct.data(), ct.size()));
+    v.ctab = dct;
+    v.ctab_inv_h = kCoulM / p->cut_off;
+  }
   if (have_sp) {
     int ns = d->special_offsets[n];
     A(s->upload(&v.sp_off, d->special_offsets, (size_t)n + 1));
@@ -629,10 +767,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   A(s->dalloc(&v.misfit_count, 1));
   A(s->dalloc(&v.spos, n));
   A(s->dalloc(&v.stype, n));
-  A(s->dalloc(&v.nbr_num, 3 * (size_t)n));
-  A(s->dalloc(&v.vir_near, 6 * (size_t)n));
-  A(s->dalloc(&v.e_lj, n));
-  A(s->dalloc(&v.e_coul, n));
+  A(s->dalloc(&v.nbr_num, 9 * (size_t)n));
   A(s->dalloc(&v.vir_k, 6 * (size_t)n));
   A(s->dalloc(&v.status, 1));
   if (rc == RBMD_OK) A(ensure_nbr(s, 0));
@@ -737,17 +872,19 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     s->nfree = (int)fr.size();
     if (s->nfree) A(s->upload(&s->free_dev, fr.data(), fr.size()));
   }
-  // reduction rows
-  int nblk = (n + kBlock - 1) / kBlock;
-  int pblk = (n * kLanes + kBlock - 1) / kBlock;  // pair kernel: kLanes threads per atom
-  int sblk = (std::max(s->nmol, 1) + kBlock - 1) / kBlock;
-  s->row_pair = 0;
-  s->row_k = pblk;
-  s->row_bond = pblk + nblk;
-  s->row_ke = pblk + 2 * nblk;
-  s->row_shake = pblk + 3 * nblk;
-  s->nrows = pblk + 3 * nblk + sblk;
-  A(s->dalloc(&s->rows, (size_t)s->nrows * S_NSLOT));
+  // reduction rows: one segment per producer, sized for its largest launch
+  {
+    const int nblk = (n + kBlock - 1) / kBlock;
+    const int cap[NSEG] = {std::max(pair_blocks(v), force_cells_blocks(v)), kforce_blocks(v), nblk, nblk,
+                           (std::max(s->nmol, 1) + kBlock - 1) / kBlock};
+    size_t off = 0;
+    for (int g = 0; g < NSEG; ++g) {
+      s->seg_off[g] = off;
+      off += (size_t)cap[g] * kSegWidth[g];
+    }
+    s->rows_len = off;
+    A(s->dalloc(&s->rows, off));
+  }
   A(s->dalloc(&s->nhc, 1));
   A(s->dalloc(&s->shake_iter, 1));
   // k space
@@ -789,7 +926,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   launch_fill_int(v.cell_count, 0, v.ncell, st);
   launch_fill_int(v.status, 0, 1, st);
   launch_fill_int(v.img, 0, 3 * n, st);
-  launch_fill_double(s->rows, 0.0, s->nrows * S_NSLOT, st);
+  launch_fill_double(s->rows, 0.0, (int)s->rows_len, st);
   launch_fill_double(v.fkx, 0.0, n, st);
   launch_fill_double(v.fky, 0.0, n, st);
   launch_fill_double(v.fkz, 0.0, n, st);
@@ -957,7 +1094,6 @@ rbmd_status rbmd_neighbors_build(rbmd_sys* s, int32_t cap) {
   if (!s->locator_built) RC_TRY(rbmd_locator_build(s));
   RC_TRY(ensure_nbr(s, cap));
   s->ctx->run("nbr_build", [&] { launch_nbr_build(s->v, s->ctx->stream); });
-  s->ctx->run("nbr_misfit", [&] { launch_nbr_misfit(s->v, s->ctx->stream); });
   s->nbr_built = true;
   return check_launch("neighbors_build");
 }
@@ -968,7 +1104,7 @@ rbmd_status rbmd_neighbors_export(rbmd_sys* s, int32_t cap, int32_t* ids, double
   if (cap != s->export_cap) return fail(RBMD_ERR_ARG, "cap %d differs from the built width %d", cap, s->export_cap);
   int n = s->n;
   hipStream_t st = s->ctx->stream;
-  std::vector<int> nbr((size_t)s->v.cap * n), cnt(3 * (size_t)n), pid(n), rank(n);
+  std::vector<int> nbr((size_t)s->v.cap * n), cnt(9 * (size_t)n), pid(n), rank(n);
   HIP_TRY(hipMemcpyAsync(nbr.data(), s->v.nbr, nbr.size() * 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(cnt.data(), s->v.nbr_num, cnt.size() * 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(pid.data(), s->v.point_ids, n * 4, hipMemcpyDeviceToHost, st));
@@ -988,7 +1124,7 @@ rbmd_status rbmd_neighbors_export(rbmd_sys* s, int32_t cap, int32_t* ids, double
         over = true;
         return;
       }
-      int e = nbr[nbr_index(slot, i, n)];
+      int e = nbr[nbr_index(slot, i, n, s->v.lanes_lg)];
       int js = e >> kShiftBits;
       int code = e & ((1 << kShiftBits) - 1);
       int sh[3] = {(code & 7) - 3, ((code >> 3) & 7) - 3, ((code >> 6) & 7) - 3};
@@ -1036,32 +1172,43 @@ rbmd_status rbmd_neighbors_count(rbmd_sys* s, int64_t* entries) {
 
 rbmd_status rbmd_force_real(rbmd_sys* s, double* force, double* vir, double* elj, double* ecoul) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
-  if (!s->nbr_built) RC_TRY(rbmd_neighbors_build(s, s->v.cap));
+  if (s->fused) {
+    if (!s->locator_built) RC_TRY(rbmd_locator_build(s));
+  } else if (!s->nbr_built) {
+    RC_TRY(rbmd_neighbors_build(s, s->v.cap));
+  }
   hipStream_t st = s->ctx->stream;
-  s->ctx->run("pair", [&] { launch_pair(s->v, st, 1, 1, s->rows + (size_t)s->row_pair * S_NSLOT, nullptr); });
+  launch_real(s, 1, 1);
   RC_TRY(check_launch("force_real"));
-  int n = s->n;
-  std::vector<double> fx(n), fy(n), fz(n), vv(6 * (size_t)n), el(n), ec(n);
+  const int n = s->n, ns = s->v.nslab;
+  std::vector<double> fx((size_t)ns * n), fy((size_t)ns * n), fz((size_t)ns * n), vv(6 * (size_t)ns * n),
+      el((size_t)ns * n), ec((size_t)ns * n);
   std::vector<int> rank(n);
-  HIP_TRY(hipMemcpyAsync(fx.data(), s->v.fnx, n * 8, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(fy.data(), s->v.fny, n * 8, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(fz.data(), s->v.fnz, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fx.data(), s->v.fnx, fx.size() * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fy.data(), s->v.fny, fy.size() * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fz.data(), s->v.fnz, fz.size() * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(vv.data(), s->v.vir_near, vv.size() * 8, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(el.data(), s->v.e_lj, n * 8, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(ec.data(), s->v.e_coul, n * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(el.data(), s->v.e_lj, el.size() * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(ec.data(), s->v.e_coul, ec.size() * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(rank.data(), s->v.rank, n * 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  // slab partials added in slab order, as the integrator does
+  auto slab_sum = [&](const std::vector<double>& a, size_t i, int stride, int k) {
+    double t = a[i * stride + k];
+    for (int sl = 1; sl < ns; ++sl) t += a[((size_t)sl * n + i) * stride + k];
+    return t;
+  };
   for (int a = 0; a < n; ++a) {
-    int i = rank[a];
+    const size_t i = rank[a];
     if (force) {
-      force[3 * a] = fx[i];
-      force[3 * a + 1] = fy[i];
-      force[3 * a + 2] = fz[i];
+      force[3 * a] = slab_sum(fx, i, 1, 0);
+      force[3 * a + 1] = slab_sum(fy, i, 1, 0);
+      force[3 * a + 2] = slab_sum(fz, i, 1, 0);
     }
     if (vir)
-      for (int k = 0; k < 6; ++k) vir[6 * a + k] = vv[6 * (size_t)i + k];
-    if (elj) elj[a] = s->v.rbl ? 0.0 : el[i];
-    if (ecoul) ecoul[a] = s->v.rbl ? 0.0 : ec[i];
+      for (int k = 0; k < 6; ++k) vir[6 * a + k] = slab_sum(vv, i, 6, k);
+    if (elj) elj[a] = s->v.rbl ? 0.0 : slab_sum(el, i, 1, 0);
+    if (ecoul) ecoul[a] = s->v.rbl ? 0.0 : slab_sum(ec, i, 1, 0);
   }
   int stt = 0;
   RC_TRY(read_status(s, &stt));
@@ -1131,7 +1278,7 @@ static rbmd_status kforce_out(rbmd_sys* s, int ewald, double* force, double* vir
   s->ctx->run("rho_partial", [&] { launch_rho_partial(s->v, k, st); });
   s->ctx->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, st); });
   s->ctx->run("kforce", [&] {
-    launch_kforce(s->v, k, st, ewald, s->p.alpha, 1, 1, s->rows + (size_t)s->row_k * S_NSLOT, nullptr);
+    launch_kforce(s->v, k, st, ewald, s->p.alpha, 1, 1, s->seg_rows(SEG_K), &s->seg_used[SEG_K]);
   });
   RC_TRY(check_launch("kforce"));
   int n = s->n;
@@ -1194,14 +1341,15 @@ rbmd_status rbmd_force_bonded(rbmd_sys* s, double* force, double* eb, double* ea
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
   hipStream_t st = s->ctx->stream;
   int nblk = 0;
-  s->ctx->run("bonded", [&] { launch_bonded(s->v, s->iv, st, 1, s->rows + (size_t)s->row_bond * S_NSLOT, &nblk); });
+  s->ctx->run("bonded", [&] { launch_bonded(s->v, s->iv, st, 1, s->seg_rows(SEG_BOND), &nblk); });
+  s->seg_used[SEG_BOND] = nblk;
   RC_TRY(check_launch("bonded"));
   int n = s->n;
-  std::vector<double> fx(n), fy(n), fz(n), rows((size_t)nblk * S_NSLOT);
+  std::vector<double> fx(n), fy(n), fz(n), rows((size_t)nblk * kRowBond);
   HIP_TRY(hipMemcpyAsync(fx.data(), s->v.fbx, n * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(fy.data(), s->v.fby, n * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(fz.data(), s->v.fbz, n * 8, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(rows.data(), s->rows + (size_t)s->row_bond * S_NSLOT, rows.size() * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(rows.data(), s->seg_rows(SEG_BOND), rows.size() * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   if (force)
     for (int i = 0; i < n; ++i) {
@@ -1211,8 +1359,8 @@ rbmd_status rbmd_force_bonded(rbmd_sys* s, double* force, double* eb, double* ea
     }
   double b = 0, a = 0;
   for (int r = 0; r < nblk; ++r) {
-    b += rows[(size_t)r * S_NSLOT + S_EBOND];
-    a += rows[(size_t)r * S_NSLOT + S_EANGLE];
+    b += rows[(size_t)r * kRowBond + (S_EBOND - S_VIR_BOND)];
+    a += rows[(size_t)r * kRowBond + (S_EANGLE - S_VIR_BOND)];
   }
   if (eb) *eb = b;
   if (ea) *ea = a;
@@ -1235,9 +1383,9 @@ rbmd_status rbmd_md_init(rbmd_sys* s) {
   RC_TRY(kvec_for_step(s, 0, &kv));
   compute_force(s, kv, 1);
   c->run("second_half", [&] {
-    launch_second_half(s->v, s->iv, st, 0, has_k(s), has_bonded(s), s->rows + (size_t)s->row_ke * S_NSLOT, nullptr);
+    launch_second_half(s->v, s->iv, st, 0, has_k(s), has_bonded(s), s->seg_rows(SEG_KE), &s->seg_used[SEG_KE]);
   });
-  c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, s->rows, s->nrows, 3, st); });
+  c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 3, st); });
   RC_TRY(check_launch("md_init"));
   s->step = 0;
   s->initialized = true;
@@ -1259,23 +1407,23 @@ static rbmd_status md_one_step(rbmd_sys* s) {
   const int tc = s->p.temp_ctrl;
   const bool nh = tc == RBMD_TEMP_NOSE_HOOVER || tc == RBMD_TEMP_NONE;
   const bool shake = shake_on(s);
-  double* rows = s->rows;
-  c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows, s->nrows, nh ? 1 : 5, st); });
+  c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, false), nh ? 1 : 5, st); });
   const double* scale = (tc == RBMD_TEMP_NOSE_HOOVER) ? &s->nhc->factor : nullptr;
   c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, 1); });
   build_cells(s, false);
   if (shake) {
     launch_fill_int(s->shake_iter, 0, 1, st);
-    c->run("shake", [&] { launch_shake(s->v, s->iv, st, rows + (size_t)s->row_shake * S_NSLOT, s->shake_iter); });
+    c->run("shake", [&] { launch_shake(s->v, s->iv, st, s->seg_rows(SEG_SHAKE), s->shake_iter); });
+    s->seg_used[SEG_SHAKE] = (s->nmol + kBlock - 1) / kBlock;
     if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 1);
   }
   gather_and_list(s);
   compute_force(s, kv, 1);
   c->run("second_half", [&] {
-    launch_second_half(s->v, s->iv, st, 1, has_k(s), has_bonded(s), rows + (size_t)s->row_ke * S_NSLOT, nullptr);
+    launch_second_half(s->v, s->iv, st, 1, has_k(s), has_bonded(s), s->seg_rows(SEG_KE), &s->seg_used[SEG_KE]);
   });
   if (nh) {
-    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows, s->nrows, 2, st); });
+    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 2, st); });
     if (tc == RBMD_TEMP_NOSE_HOOVER && !(shake && s->nfree == 0))
       c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
     if (shake) {
@@ -1286,9 +1434,9 @@ static rbmd_status md_one_step(rbmd_sys* s) {
     if (shake) {
       c->run("rattle", [&] { launch_rattle(s->v, s->iv, st); });
       if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 0);
-      c->run("ke_rows", [&] { launch_ke_rows(s->v, s->iv, st, rows + (size_t)s->row_ke * S_NSLOT); });
+      c->run("ke_rows", [&] { launch_ke_rows(s->v, s->iv, st, s->seg_rows(SEG_KE)); });
     }
-    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows, s->nrows, 4, st); });
+    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 4, st); });
     c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
   }
   s->step = step;

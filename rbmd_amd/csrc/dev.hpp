@@ -10,15 +10,18 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kBlock = 256;     // threads per workgroup (4 wavefronts of 64)
 constexpr int kWave = 64;       // CDNA wavefront width
 constexpr int kChunk = 64;      // i-atoms per neighbour-build work item (one wavefront)
-constexpr int kMaxSpecial = 8;  // special-list entries held in registers per atom
+constexpr int kMaxSpecial = 8;  // special-list entries held in registers per atom (RBL kernels)
 constexpr int kShiftBits = 9;   // 3 bits per axis image shift in packed neighbour entries
-constexpr int kLanes = 4;       // lanes sharing one atom's neighbour list in the pair kernel
+constexpr int kMaxLanes = 16;   // most lanes sharing one atom's neighbour list
+constexpr int kCoulM = 64;      // Coulomb table intervals over [0, rc]
+constexpr int kCoulD = 7;       // polynomial degree per interval
+constexpr int kMaxTypes = 16;   // type-pair tables held in LDS up to this many types
 
-// Neighbour-list slot layout: slot k of atom i (sorted order) lives at
-// ((k / kLanes) * n + i) * kLanes + k % kLanes, so the kLanes lanes that share
-// atom i, and consecutive atoms, read consecutive words.
-__host__ __device__ __forceinline__ size_t nbr_index(int slot, int i, int n) {
-  return ((size_t)(slot / kLanes) * n + i) * kLanes + (slot % kLanes);
+// Neighbour-list slot layout for L = 2^lg lanes per atom: slot k of atom i
+// (sorted order) lives at ((k >> lg) * n + i) << lg | (k & (L-1)), so the L
+// lanes that share atom i, and consecutive atoms, read consecutive words.
+__host__ __device__ __forceinline__ size_t nbr_index(int slot, int i, int n, int lg) {
+  return ((((size_t)(slot >> lg)) * n + i) << lg) | (size_t)(slot & ((1 << lg) - 1));
 }
 
 // Everything a kernel needs to know about one system.  Passed by value.
@@ -29,14 +32,18 @@ struct SysView {
   int dims[3];
   int nc;           // stencil half width (ceil(rc / rs))
   int cap;          // neighbour-list slots per atom (VERLET: nseg * seg_cap)
-  int nseg;         // VERLET: one list segment per x-slab of the stencil (2nc+1); RBL: 1
-  int seg_cap;      // slots per segment (multiple of kLanes)
+  int nseg;         // VERLET: one list segment per (x, y) column of the 3x3x3 stencil (9); RBL: 1
+  int seg_cap;      // slots per segment (multiple of the lane count)
+  int lanes_lg;     // log2 of the lanes per atom in the pair kernel (2..4)
+  int sp_max;       // longest special list
   int n_items;      // capacity of the work-item list
   int weighted;     // special-bond weights in the pair term
   int lj_only;      // LJ/CUT force field
   int rbl;          // random-batch list
   int erfc_table;   // step-table erfc (alpha = 0.1)
   int rs_num, pice_num;
+  int nslab;        // real-space partials per atom in fn*, vir_near, e_lj, e_coul (fused path 3, else 1)
+  int use_min_image;  // some box edge < 2 rc: the pair vector may need the minimum image
   double lo[3], hi[3], box[3], dxdydz[3];
   double volume, cut, cut2, rs, rs2, alpha, sqrt_alpha, qqr2e;
   // per atom, original order (SoA)
@@ -46,6 +53,12 @@ struct SysView {
   int *type, *img;
   // per type pair tables [ntypes*ntypes]
   double *eps_ij, *sig6_ij;
+  // Coulomb radial table: kCoulM intervals of width h over [0, rc]; per
+  // interval the degree-7 coefficients (in u = r/h - k - 1/2) of
+  // P(r) = erfc(beta r) + 2 beta/sqrt(pi) r e^{-beta^2 r^2} (so G(r)/r = P / r^3)
+  // and of Q(r) = erfc(beta r)
+  const double* ctab;  // [kCoulM][2][kCoulD + 1]
+  double ctab_inv_h;
   // special lists (CSR) and molecule groups
   int *sp_off, *sp_ids;
   double *sp_w;
@@ -69,7 +82,7 @@ struct SysView {
   int lds_bins;     // binning kernels aggregate counts in LDS (ncell small enough)
   int* nbr_num;     // VERLET [nseg][n] counts per segment; RBL [2n] (core, shell kept)
   // per-atom force components (original order)
-  double *fnx, *fny, *fnz;  // real space, sorted order
+  double *fnx, *fny, *fnz;  // real space, sorted order, [nslab][n] partials added in order
   double *fkx, *fky, *fkz;  // k space
   double *fbx, *fby, *fbz;  // bonded
   // per-atom optional outputs (sorted order for real space)
@@ -81,7 +94,11 @@ struct SysView {
   int* status;
 };
 
-// Reduction slots of the scalar block (doubles); see api.cpp for the layout.
+// Thermo scalars reduced from per-workgroup partial rows.  Each producer
+// writes rows of its own width holding consecutive slots:
+//   pair   S_ELJ .. S_VIR_NEAR+5  (8)     k space  S_VIR_K .. +5   (6)
+//   bonded S_VIR_BOND .. S_EANGLE (14)    KE       S_KE            (1)
+//   SHAKE  S_VIR_SHAKE .. +5      (6)
 enum ScalarSlot {
   S_KE = 0,
   S_ELJ = 1,
@@ -90,10 +107,22 @@ enum ScalarSlot {
   S_VIR_K = 9,      // 6
   S_VIR_BOND = 15,  // 6
   S_VIR_ANGLE = 21, // 6
-  S_VIR_SHAKE = 27, // 6
-  S_EBOND = 33,
-  S_EANGLE = 34,
-  S_NSLOT = 40
+  S_EBOND = 27,
+  S_EANGLE = 28,
+  S_VIR_SHAKE = 29, // 6
+  S_NSLOT = 35
+};
+constexpr int kRowPair = 8, kRowK = 6, kRowBond = 14, kRowKe = 1, kRowShake = 6;
+
+// One producer's rows: nrows rows of `width` doubles, row k -> slot slot0 + k;
+// `groups` is how many thread groups stride the rows in the reduction.
+struct RowSeg {
+  const double* rows;
+  int nrows, width, slot0, groups;
+};
+struct RowsDesc {
+  RowSeg seg[5];
+  int nseg;
 };
 
 // ---------------------------------------------------------------- cells
@@ -103,13 +132,16 @@ void launch_scatter(const SysView& s, hipStream_t st);
 void launch_cell_sort(const SysView& s, hipStream_t st);
 void launch_gather(const SysView& s, hipStream_t st);
 void launch_nbr_build(const SysView& s, hipStream_t st);
-void launch_nbr_misfit(const SysView& s, hipStream_t st);
 void launch_fill_int(int* p, int v, int n, hipStream_t st);
 void launch_fill_double(double* p, double v, int n, hipStream_t st);
 
 // ---------------------------------------------------------------- pair
-// partials: [nblocks][S_NSLOT] block sums (may be null when thermo == 0)
+// partials: [nblocks][kRowPair] block sums (may be null when thermo == 0)
 void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, double* partials, int* nblocks);
+int pair_blocks(const SysView& s);
+// fused cell-stencil real-space force (no stored list); fn* get 3 slab partials
+void launch_force_cells(const SysView& s, hipStream_t st, int thermo, int peratom, double* partials, int* nblocks);
+int force_cells_blocks(const SysView& s);
 
 // ---------------------------------------------------------------- k space
 struct KView {
@@ -124,6 +156,7 @@ void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st);
 void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st);
 void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, int thermo,
                    int peratom, double* partials, int* nblocks);
+int kforce_blocks(const SysView& s);
 
 // ---------------------------------------------------------------- integrate
 struct IntView {
@@ -171,6 +204,6 @@ struct NhcParams {
 };
 // phase 1: step++ and NHC half step (InitialIntegrate); 2: ComputeTempe + pressure + NHC
 // (FinalIntegrate); 3: SetUp; 4: Berendsen/rescale end of step; 5: step++ only
-void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const double* rows, int nrows, int phase, hipStream_t st);
+void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st);
 
 }  // namespace rbmd

@@ -17,6 +17,7 @@
 // kernel's per-lane reads of slot k are coalesced.
 #include "dev.hpp"
 #include "reduce.hpp"
+#include "cells.hpp"
 
 namespace rbmd {
 
@@ -36,13 +37,6 @@ void launch_fill_int(int* p, int v, int n, hipStream_t st) {
 void launch_fill_double(double* p, double v, int n, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_double, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, p, v, n);
-}
-
-// InWhichCell (ExecPointLocator.h:271-274): unclamped trunc
-__device__ __forceinline__ int cell_axis(double x, double lo, double d) {
-  double t = (x - lo) / d;
-  if (!(t == t)) return 0;
-  return (int)fmax(fmin(t, 2.0e9), -2.0e9);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bin(SysView s, const double* __restrict__ x, const double* __restrict__ y,
@@ -217,73 +211,75 @@ void launch_gather(const SysView& s, hipStream_t st) {
   hipLaunchKernelGGL(k_gather, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
 }
 
-__device__ __forceinline__ int wrap_axis(int v, int dims) { return (v + dims) % dims; }
-
-__device__ __forceinline__ int pack_entry(int j, int sa, int sb, int sc) {
-  return (j << kShiftBits) | ((sa + 3) | ((sb + 3) << 3) | ((sc + 3) << 6));
-}
-
-// wavefront-local LDS ordering (writes of all lanes visible before reads)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// One stencil cell seen from cell (ca,cb,cc) at offset (a,b,e): wrapped cell,
-// image offset dxdydz * (wrapped - unwrapped) (PeriodicIndexOffset /
-// PeriodicCoordOffset, ExecPointLocator.h:298-319) and the packed shift code.
-struct StencilCell {
-  int js, je, sa, sb, sc;
-  double ox, oy, oz;
-  bool ok;
-};
-
-__device__ __forceinline__ StencilCell stencil_cell(const SysView& s, int ca, int cb, int cc, int a, int b, int e) {
-  StencilCell t;
-  int na = ca + a, nb = cb + b, ne = cc + e;
-  int wa = wrap_axis(na, s.dims[0]), wb = wrap_axis(nb, s.dims[1]), we = wrap_axis(ne, s.dims[2]);
-  t.ok = wa >= 0 && wb >= 0 && we >= 0;
-  if (!t.ok) {
-    t.js = t.je = 0;
-    return t;
-  }
-  t.ox = s.dxdydz[0] * (double)(wa - na);
-  t.oy = s.dxdydz[1] * (double)(wb - nb);
-  t.oz = s.dxdydz[2] * (double)(we - ne);
-  t.sa = (wa - na) / s.dims[0];
-  t.sb = (wb - nb) / s.dims[1];
-  t.sc = (we - ne) / s.dims[2];
-  if (t.sa < -3 || t.sa > 3 || t.sb < -3 || t.sb > 3 || t.sc < -3 || t.sc > 3) {
-    t.ok = false;
-    t.js = t.je = 0;
-    return t;
-  }
-  int wcell = wa + wb * s.dims[0] + we * s.dims[0] * s.dims[1];
-  t.js = s.cell_start[wcell];
-  t.je = s.cell_start[wcell + 1];
-  return t;
-}
-
-// VERLET list build.  One workgroup per work item (64 i-atoms of one cell),
-// one wavefront per x-slab a of the 3x3x3 stencil; wavefront a writes list
-// segment a + 1, so the concatenated segments are in the reference's order
-// (x offset outermost, then y, then z, then the cell's sorted order).
+// VERLET list build.  One workgroup per work item (64 i-atoms of one cell);
+// wavefront w = 3 (a+1) + (b+1) owns the column (a, b) of the 3x3x3 stencil
+// (its three cells e = -1, 0, 1) and writes list segment w, so the
+// concatenated segments follow the reference's order (x offset outermost,
+// then y, then z, then the cell's sorted order).
 //
-// The 9 cells of a slab form one candidate stream.  Each round, every lane
-// stages one candidate (coalesced load, image shift applied); candidates that
-// are farther than the cutoff from the i-chunk's bounding box are dropped by a
+// The three cells of a column form one candidate stream.  Each round, every
+// lane holds one candidate (coalesced load, image shift applied; the next
+// round's load is issued before this round is processed).  Candidates farther
+// than the cutoff from the i-chunk's bounding box are dropped by a
 // wave-uniform ballot and the rest are compacted into LDS.  Every lane then
 // tests the compacted candidates against its own i-atom with the reference's
 // FP64 predicate in a branch-free loop that only sets bits of an accept mask,
 // and finally writes the accepted entries in stream order.
-constexpr int kStageRounds = 1;
+//
+// Workgroups past the work items take the misfit atoms (whose current cell is
+// not the binned one, e.g. after SHAKE): one workgroup per misfit atom, the 9
+// wavefronts again own the 9 columns, lanes test 64 candidates at a time and
+// accepted ones are written at their ballot prefix position (stream order).
+constexpr int kColWaves = 9;
 
-__global__ __launch_bounds__(3 * kWave) void k_nbr_build_verlet(SysView s) {
-  __shared__ double4 stage[3][kWave];
-  __shared__ int stage_e[3][kWave];
+__device__ void misfit_block(const SysView& s, int wave, int lane) {
+  const int count = s.misfit_count[0];
+  const int a = wave / 3 - 1, b = wave % 3 - 1;
+  const int seg0 = wave * s.seg_cap;
+  for (int m = blockIdx.x - s.n_items; m < count; m += kMisfitBlocks) {
+    const int i = __builtin_amdgcn_readfirstlane(s.misfit_list[m]);
+    const double4 pi = s.spos[i];
+    const int ca = cell_axis(pi.x, s.lo[0], s.dxdydz[0]);
+    const int cb = cell_axis(pi.y, s.lo[1], s.dxdydz[1]);
+    const int cc = cell_axis(pi.z, s.lo[2], s.dxdydz[2]);
+    Column col;
+    make_column(s, ca, cb, cc, a, b, lane, col);
+    const int total = col.total;
+    int cnt = 0;
+    for (int base = 0; base < total; base += kWave) {
+      const int t = base + lane;
+      bool ok = false;
+      int entry = 0;
+      if (t < total) {
+        double4 pj;
+        COLUMN_CANDIDATE(t, pj, entry);
+        double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
+        double d2 = rx * rx + ry * ry + rz * rz;
+        ok = s.cut2 - d2 > 0.0001 && d2 > 0.0001;
+      }
+      const unsigned long long bal = __ballot(ok);
+      if (ok) {
+        int slot = cnt + __popcll(bal & ((1ull << lane) - 1ull));
+        if (slot < s.seg_cap)
+          s.nbr[nbr_index(seg0 + slot, i, s.n, s.lanes_lg)] = entry;
+        else
+          atomicOr(s.status, 4);
+      }
+      cnt += __popcll(bal);
+    }
+    if (lane == 0) s.nbr_num[wave * s.n + i] = cnt < s.seg_cap ? cnt : s.seg_cap;
+  }
+}
+
+__global__ __launch_bounds__(kColWaves * kWave) void k_nbr_build_verlet(SysView s) {
+  __shared__ double4 stage[kColWaves][kWave];
+  __shared__ int stage_e[kColWaves][kWave];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
+  if ((int)blockIdx.x >= s.n_items) {
+    misfit_block(s, wave, lane);
+    return;
+  }
   const int item_idx = blockIdx.x;
   if (item_idx >= s.n_items_dev[0]) return;
   const int item = __builtin_amdgcn_readfirstlane(s.items[item_idx]);
@@ -297,7 +293,7 @@ __global__ __launch_bounds__(3 * kWave) void k_nbr_build_verlet(SysView s) {
   double4 pi = make_double4(0, 0, 0, 0);
   if (active) {
     pi = s.spos[i];
-    if (s.misfit[i]) active = false;  // handled by k_nbr_misfit_verlet
+    if (s.misfit[i]) active = false;  // a misfit workgroup lists it
   }
   // bounding box of the active i-atoms (wave reductions)
   double bl[3], bh[3];
@@ -316,43 +312,24 @@ __global__ __launch_bounds__(3 * kWave) void k_nbr_build_verlet(SysView s) {
   const int ca = c % s.dims[0];
   const int cb = (c / s.dims[0]) % s.dims[1];
   const int cc = c / (s.dims[0] * s.dims[1]);
-  const int a = wave - 1;
+  const int a = wave / 3 - 1, b = wave % 3 - 1;
   const int seg0 = wave * s.seg_cap;
-  // the slab's 9 cells as one stream: per-cell start, count prefix, offset, code
-  int cs[9], pre[10], code[9];
-  double ofx[9], ofy[9], ofz[9];
-  pre[0] = 0;
-  for (int q = 0; q < 9; ++q) {
-    StencilCell t = stencil_cell(s, ca, cb, cc, a, q / 3 - 1, q % 3 - 1);
-    if (!t.ok) {
-      if (lane == 0) atomicOr(s.status, 4);
-      t.js = t.je = 0;
-      t.sa = t.sb = t.sc = 0;
-      t.ox = t.oy = t.oz = 0;
-    }
-    cs[q] = t.js;
-    pre[q + 1] = pre[q] + (t.je - t.js);
-    code[q] = pack_entry(0, t.sa, t.sb, t.sc);
-    ofx[q] = t.ox;
-    ofy[q] = t.oy;
-    ofz[q] = t.oz;
-  }
-  const int total = pre[9];
+  Column col;
+  make_column(s, ca, cb, cc, a, b, lane, col);
+  const int total = col.total;
   const double cut2 = s.cut2;
   int cnt = 0;
+  // software pipeline: the candidate of the next round is loaded ahead
+  double4 nxt = make_double4(0, 0, 0, 0);
+  int nxt_e = 0;
+  if (lane < total) COLUMN_CANDIDATE(lane, nxt, nxt_e);
   for (int base = 0; base < total; base += kWave) {
     const int t = base + lane;
+    const double4 pj = nxt;
+    const int entry = nxt_e;
+    if (t + kWave < total) COLUMN_CANDIDATE(t + kWave, nxt, nxt_e);
     bool keep = false;
-    double4 pj = make_double4(0, 0, 0, 0);
-    int entry = 0;
     if (t < total) {
-      int q = 0;
-#pragma unroll
-      for (int u = 1; u < 9; ++u) q += (t >= pre[u]) ? 1 : 0;
-      int j = cs[q] + (t - pre[q]);
-      double4 p = s.spos[j];
-      pj = make_double4(p.x - ofx[q], p.y - ofy[q], p.z - ofz[q], 0.0);
-      entry = (j << kShiftBits) | code[q];
       double dx = fmax(0.0, fmax(bl[0] - pj.x, pj.x - bh[0]));
       double dy = fmax(0.0, fmax(bl[1] - pj.y, pj.y - bh[1]));
       double dz = fmax(0.0, fmax(bl[2] - pj.z, pj.z - bh[2]));
@@ -368,7 +345,7 @@ __global__ __launch_bounds__(3 * kWave) void k_nbr_build_verlet(SysView s) {
     }
     wave_sync();
     unsigned long long acc = 0;
-#pragma unroll 8
+#pragma unroll 4
     for (int u = 0; u < kc; ++u) {
       const double4 q = stage[wave][u];
       const double rx = q.x - pi.x;
@@ -383,7 +360,7 @@ __global__ __launch_bounds__(3 * kWave) void k_nbr_build_verlet(SysView s) {
       const int u = __builtin_ctzll(acc);
       acc &= acc - 1;
       if (cnt < s.seg_cap)
-        s.nbr[nbr_index(seg0 + cnt, i, s.n)] = stage_e[wave][u];
+        s.nbr[nbr_index(seg0 + cnt, i, s.n, s.lanes_lg)] = stage_e[wave][u];
       else
         atomicOr(s.status, 4);
       cnt++;
@@ -391,61 +368,6 @@ __global__ __launch_bounds__(3 * kWave) void k_nbr_build_verlet(SysView s) {
     wave_sync();
   }
   if (active) s.nbr_num[wave * s.n + i] = cnt < s.seg_cap ? cnt : s.seg_cap;
-}
-
-// Atoms whose current cell differs from the binned one (after SHAKE moves an
-// atom across a cell face, or at x == hi): the stencil is centred on
-// InWhichCell(p_i) as in the reference, so such an atom cannot share the
-// work item of its binned cell.  One wavefront per misfit atom: lanes test 64
-// candidates at a time and the accepted ones are written at their ballot
-// prefix position, which keeps the reference's traversal order.
-__global__ __launch_bounds__(kBlock) void k_nbr_misfit_verlet(SysView s) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-  const int nw = gridDim.x * blockDim.x / kWave;
-  const int count = s.misfit_count[0];
-  for (int m = gw; m < count; m += nw) {
-    const int i = __builtin_amdgcn_readfirstlane(s.misfit_list[m]);
-    const double4 pi = s.spos[i];
-    const int ca = cell_axis(pi.x, s.lo[0], s.dxdydz[0]);
-    const int cb = cell_axis(pi.y, s.lo[1], s.dxdydz[1]);
-    const int cc = cell_axis(pi.z, s.lo[2], s.dxdydz[2]);
-    for (int a = -1; a <= 1; ++a) {
-      const int seg0 = (a + 1) * s.seg_cap;
-      int cnt = 0;
-      for (int b = -1; b <= 1; ++b)
-        for (int e = -1; e <= 1; ++e) {
-          StencilCell t = stencil_cell(s, ca, cb, cc, a, b, e);
-          if (!t.ok) {
-            if (lane == 0) atomicOr(s.status, 1);
-            continue;
-          }
-          const int code = pack_entry(0, t.sa, t.sb, t.sc);
-          for (int base = t.js; base < t.je; base += kWave) {
-            const int j = base + lane;
-            bool ok = false;
-            if (j < t.je) {
-              double4 pj = s.spos[j];
-              double rx = (pj.x - t.ox) - pi.x;
-              double ry = (pj.y - t.oy) - pi.y;
-              double rz = (pj.z - t.oz) - pi.z;
-              double d2 = rx * rx + ry * ry + rz * rz;
-              ok = s.cut2 - d2 > 0.0001 && d2 > 0.0001;
-            }
-            const unsigned long long bal = __ballot(ok);
-            if (ok) {
-              int slot = cnt + __popcll(bal & ((1ull << lane) - 1ull));
-              if (slot < s.seg_cap)
-                s.nbr[nbr_index(seg0 + slot, i, s.n)] = (j << kShiftBits) | code;
-              else
-                atomicOr(s.status, 4);
-            }
-            cnt += __popcll(bal);
-          }
-        }
-      if (lane == 0) s.nbr_num[(a + 1) * s.n + i] = cnt < s.seg_cap ? cnt : s.seg_cap;
-    }
-  }
 }
 
 // RBL list state: core slots [0, rs_num), kept shell slots from rs_num (every
@@ -459,7 +381,7 @@ __device__ __forceinline__ void rbl_consider(const SysView& s, RblState& L, int 
     if (L.shell_seen % s.pice_num == 0) {
       int slot = s.rs_num + L.shell_kept;
       if (slot < s.cap)
-        s.nbr[nbr_index(slot, i, s.n)] = entry;
+        s.nbr[nbr_index(slot, i, s.n, s.lanes_lg)] = entry;
       else
         atomicOr(s.status, 4);
       L.shell_kept++;
@@ -467,7 +389,7 @@ __device__ __forceinline__ void rbl_consider(const SysView& s, RblState& L, int 
     L.shell_seen++;
   } else if (s.rs2 - d2 > 0.0001 && d2 > 0.0001) {
     if (L.core < s.rs_num)
-      s.nbr[nbr_index(L.core, i, s.n)] = entry;
+      s.nbr[nbr_index(L.core, i, s.n, s.lanes_lg)] = entry;
     else
       atomicOr(s.status, 4);
     L.core++;
@@ -570,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_misfit_rbl(SysView s) {
             if (is_core) {
               int slot = core + __popcll(bc & below);
               if (slot < s.rs_num)
-                s.nbr[nbr_index(slot, i, s.n)] = entry;
+                s.nbr[nbr_index(slot, i, s.n, s.lanes_lg)] = entry;
               else
                 atomicOr(s.status, 4);
             }
@@ -581,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_misfit_rbl(SysView s) {
             if (keep) {
               int slot = s.rs_num + kept + __popcll(bk & below);
               if (slot < s.cap)
-                s.nbr[nbr_index(slot, i, s.n)] = entry;
+                s.nbr[nbr_index(slot, i, s.n, s.lanes_lg)] = entry;
               else
                 atomicOr(s.status, 4);
             }
@@ -600,19 +522,11 @@ void launch_nbr_build(const SysView& s, hipStream_t st) {
   if (s.rbl) {
     int grid = (s.n_items + (kBlock / kWave) - 1) / (kBlock / kWave);
     hipLaunchKernelGGL(k_nbr_build_rbl, dim3(grid), dim3(kBlock), 0, st, s);
+    int waves = s.n < 1024 ? s.n : 1024;
+    hipLaunchKernelGGL(k_nbr_misfit_rbl, dim3((waves * kWave + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
   } else {
-    hipLaunchKernelGGL(k_nbr_build_verlet, dim3(s.n_items), dim3(3 * kWave), 0, st, s);
+    hipLaunchKernelGGL(k_nbr_build_verlet, dim3(s.n_items + kMisfitBlocks), dim3(kColWaves * kWave), 0, st, s);
   }
-}
-
-// grid-stride over the misfit list: up to 1024 wavefronts
-void launch_nbr_misfit(const SysView& s, hipStream_t st) {
-  int waves = s.n < 1024 ? s.n : 1024;
-  int grid = (waves * kWave + kBlock - 1) / kBlock;
-  if (s.rbl)
-    hipLaunchKernelGGL(k_nbr_misfit_rbl, dim3(grid), dim3(kBlock), 0, st, s);
-  else
-    hipLaunchKernelGGL(k_nbr_misfit_verlet, dim3(grid), dim3(kBlock), 0, st, s);
 }
 
 }  // namespace rbmd

@@ -219,10 +219,8 @@ __global__ __launch_bounds__(kBlock) void k_shake(SysView s, IntView iv, double*
     vir[5] = l01 * r01.y * r01.z + l20 * r20.y * r20.z + l12 * r12.y * r12.z;
   }
   if (max_iter_out) atomicMax(max_iter_out, my_iter);
-  const double vals[6] = {vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
-  const int slots[6] = {S_VIR_SHAKE, S_VIR_SHAKE + 1, S_VIR_SHAKE + 2, S_VIR_SHAKE + 3, S_VIR_SHAKE + 4,
-                        S_VIR_SHAKE + 5};
-  block_write_row<6>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+  const double vals[kRowShake] = {vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
+  block_write_row<kRowShake>(vals, rows + (size_t)blockIdx.x * kRowShake);
 }
 
 // atoms outside every SHAKE triplet get x = x_old, v = v_old: the reference
@@ -386,11 +384,10 @@ __global__ __launch_bounds__(kBlock) void k_bonded(SysView s, IntView iv, double
     s.fbz[i] = f[2];
   }
   if (THERMO) {
-    const double vals[14] = {eb, ea, vb[0], vb[1], vb[2], vb[3], vb[4], vb[5], va[0], va[1], va[2], va[3], va[4], va[5]};
-    const int slots[14] = {S_EBOND,        S_EANGLE,        S_VIR_BOND,      S_VIR_BOND + 1,  S_VIR_BOND + 2,
-                           S_VIR_BOND + 3, S_VIR_BOND + 4,  S_VIR_BOND + 5,  S_VIR_ANGLE,     S_VIR_ANGLE + 1,
-                           S_VIR_ANGLE + 2, S_VIR_ANGLE + 3, S_VIR_ANGLE + 4, S_VIR_ANGLE + 5};
-    block_write_row<14>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+    // slots S_VIR_BOND .. S_VIR_ANGLE + 5, S_EBOND, S_EANGLE
+    const double vals[kRowBond] = {vb[0], vb[1], vb[2], vb[3], vb[4], vb[5], va[0],
+                                   va[1], va[2], va[3], va[4], va[5], eb,    ea};
+    block_write_row<kRowBond>(vals, rows + (size_t)blockIdx.x * kRowBond);
   }
 }
 
@@ -403,7 +400,8 @@ void launch_bonded(const SysView& s, const IntView& iv, hipStream_t st, int ther
     hipLaunchKernelGGL(k_bonded<false>, dim3(grid), dim3(kBlock), 0, st, s, iv, rows);
 }
 
-// F = F_real[rank] + F_k + F_bonded; optional second kick; kinetic energy row
+// F = F_real[rank] (slab partials in order) + F_k + F_bonded; optional second kick;
+// kinetic energy row
 __global__ __launch_bounds__(kBlock) void k_second_half(SysView s, IntView iv, int kick, int has_k, int has_b,
                                                         double* rows) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -411,6 +409,12 @@ __global__ __launch_bounds__(kBlock) void k_second_half(SysView s, IntView iv, i
   if (i < s.n) {
     int r = s.rank[i];
     double fx = s.fnx[r], fy = s.fny[r], fz = s.fnz[r];
+    for (int k = 1; k < s.nslab; ++k) {
+      const size_t o = (size_t)k * s.n + r;
+      fx += s.fnx[o];
+      fy += s.fny[o];
+      fz += s.fnz[o];
+    }
     if (has_k) {
       fx = s.fkx[i] + fx;
       fy = s.fky[i] + fy;
@@ -440,9 +444,8 @@ __global__ __launch_bounds__(kBlock) void k_second_half(SysView s, IntView iv, i
     ke = m * (vx * vx + vy * vy + vz * vz) * iv.mvv2e;
     if (!isfinite(fx + fy + fz)) atomicOr(s.status, 1);
   }
-  const double vals[1] = {ke};
-  const int slots[1] = {S_KE};
-  block_write_row<1>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+  const double vals[kRowKe] = {ke};
+  block_write_row<kRowKe>(vals, rows + (size_t)blockIdx.x * kRowKe);
 }
 
 void launch_second_half(const SysView& s, const IntView& iv, hipStream_t st, int kick, int has_k, int has_b,
@@ -456,9 +459,8 @@ __global__ __launch_bounds__(kBlock) void k_ke_rows(SysView s, IntView iv, doubl
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double ke = 0.0;
   if (i < s.n) ke = s.mass[i] * (s.vx[i] * s.vx[i] + s.vy[i] * s.vy[i] + s.vz[i] * s.vz[i]) * iv.mvv2e;
-  const double vals[1] = {ke};
-  const int slots[1] = {S_KE};
-  block_write_row<1>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+  const double vals[kRowKe] = {ke};
+  block_write_row<kRowKe>(vals, rows + (size_t)blockIdx.x * kRowKe);
 }
 
 void launch_ke_rows(const SysView& s, const IntView& iv, hipStream_t st, double* rows) {
@@ -480,10 +482,6 @@ void launch_scale_v(const SysView& s, hipStream_t st, const double* scale) {
 
 // ------------------------------------------------------------------ NHC
 
-struct RowsDesc {
-  const double* rows;
-  int nrows;
-};
 
 __device__ void nhc_integrate(NhcState& S, const NhcParams& p) {
   double T = S.tempT;
@@ -568,13 +566,36 @@ __device__ void compute_pressure(NhcState& S, const NhcParams& p, const double* 
 // lanes stride the rows, fixed-order shuffle tree), then thread 0 advances the
 // thermostat state.
 __global__ __launch_bounds__(1024) void k_nhc(NhcState* state, NhcParams p, RowsDesc rd, int phase) {
+  // All producer segments at once: segment g owns threads [base, base +
+  // groups * width); thread (group, k) strides rows group, group + groups, ...
+  // of column k (consecutive threads read consecutive doubles), then thread k
+  // of the segment adds the group partials in order.  Fixed order throughout.
+  __shared__ double part[1024];
   __shared__ double sums[S_NSLOT];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int slot = wave; slot < S_NSLOT; slot += blockDim.x / 64) {
+  const int tid = threadIdx.x;
+  if (tid < S_NSLOT) sums[tid] = 0.0;
+  int base = 0, mine = -1, mbase = 0;
+  for (int g = 0; g < rd.nseg; ++g) {
+    const int span = rd.seg[g].groups * rd.seg[g].width;
+    if (tid >= base && tid < base + span) {
+      mine = g;
+      mbase = base;
+    }
+    base += span;
+  }
+  if (mine >= 0) {
+    const RowSeg sg = rd.seg[mine];
+    const int k = (tid - mbase) % sg.width, grp = (tid - mbase) / sg.width;
     double acc = 0.0;
-    for (int r = lane; r < rd.nrows; r += 64) acc += rd.rows[(size_t)r * S_NSLOT + slot];
-    acc = wave_sum(acc);
-    if (lane == 0) sums[slot] = acc;
+    for (int r = grp; r < sg.nrows; r += sg.groups) acc += sg.rows[(size_t)r * sg.width + k];
+    part[tid] = acc;
+  }
+  __syncthreads();
+  if (mine >= 0 && tid - mbase < rd.seg[mine].width) {
+    const RowSeg sg = rd.seg[mine];
+    double acc = 0.0;
+    for (int grp = 0; grp < sg.groups; ++grp) acc += part[mbase + grp * sg.width + (tid - mbase)];
+    sums[sg.slot0 + (tid - mbase)] = acc;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -628,8 +649,7 @@ __global__ __launch_bounds__(1024) void k_nhc(NhcState* state, NhcParams p, Rows
   *state = S;
 }
 
-void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const double* rows, int nrows, int phase, hipStream_t st) {
-  RowsDesc rd{rows, nrows};
+void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st) {
   hipLaunchKernelGGL(k_nhc, dim3(1), dim3(1024), 0, st, st_dev, p, rd, phase);
 }
 

@@ -19,32 +19,43 @@ namespace rbmd {
 constexpr int kRhoAtomsPerThread = 4;
 constexpr int kRhoK = 8;  // k-vectors per workgroup in the rho pass
 
+// e^{i k.r} with k = 2 pi m / L: the phase m.(r/L) in turns is reduced to
+// [-1/2, 1/2] before sincospi, so no large-argument reduction is needed
+// (k.r reaches several hundred radians at |m| ~ 30).
+__device__ __forceinline__ void phase_sincos(double mx, double my, double mz, double sx, double sy, double sz,
+                                             double* sn, double* cs) {
+  double t = mx * sx + my * sy + mz * sz;
+  t -= rint(t);
+  sincospi(2.0 * t, sn, cs);
+}
+
+__device__ __forceinline__ double k_to_m(double k, double box) { return rint(k * box / (2.0 * kPi)); }
+
 __global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
   __shared__ double lds[kBlock / kWave][2 * kRhoK];
   const int l0 = blockIdx.y * kRhoK;
   double re[kRhoK], im[kRhoK];
 #pragma unroll
   for (int t = 0; t < kRhoK; ++t) re[t] = im[t] = 0.0;
-  double kx[kRhoK], ky[kRhoK], kz[kRhoK];
+  double mx[kRhoK], my[kRhoK], mz[kRhoK];
 #pragma unroll
   for (int t = 0; t < kRhoK; ++t) {
     int l = l0 + t;
     bool ok = l < kv.P;
-    kx[t] = ok ? kv.kvec[3 * l] : 0.0;
-    ky[t] = ok ? kv.kvec[3 * l + 1] : 0.0;
-    kz[t] = ok ? kv.kvec[3 * l + 2] : 0.0;
+    mx[t] = ok ? k_to_m(kv.kvec[3 * l], s.box[0]) : 0.0;
+    my[t] = ok ? k_to_m(kv.kvec[3 * l + 1], s.box[1]) : 0.0;
+    mz[t] = ok ? k_to_m(kv.kvec[3 * l + 2], s.box[2]) : 0.0;
   }
   const int base = blockIdx.x * kBlock * kRhoAtomsPerThread;
 #pragma unroll
   for (int a = 0; a < kRhoAtomsPerThread; ++a) {
     int i = base + a * kBlock + threadIdx.x;
     if (i < s.n) {
-      double x = s.x[i], y = s.y[i], z = s.z[i], q = s.q[i];
+      double sx = s.x[i] / s.box[0], sy = s.y[i] / s.box[1], sz = s.z[i] / s.box[2], q = s.q[i];
 #pragma unroll
       for (int t = 0; t < kRhoK; ++t) {
-        double ph = kx[t] * x + ky[t] * y + kz[t] * z;
         double sn, cs;
-        sincos(ph, &sn, &cs);
+        phase_sincos(mx[t], my[t], mz[t], sx, sy, sz, &sn, &cs);
         re[t] += q * cs;
         im[t] += q * sn;
       }
@@ -91,46 +102,57 @@ void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st) {
 }
 
 constexpr int kForceTile = 256;
+constexpr int kKLg = 2;  // 4 lanes per atom split the k list
 
-// prefactor pre_l = -4 pi / (V k^2) [* e^{-k^2/4alpha} for Ewald]
+// prefactor pre_l = -4 pi / (V k^2) [* e^{-k^2/4alpha} for Ewald]; lane `sub`
+// of an atom takes k-vectors sub, sub + 4, ... of each LDS tile
 template <bool THERMO, bool PERATOM>
 __global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewald, double alpha, double* rows) {
+  constexpr int L = 1 << kKLg;
   __shared__ double tk[kForceTile][4];
+  __shared__ double tm[kForceTile][3];
   __shared__ double tr[kForceTile][2];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t >> kKLg;
+  const int sub = t & (L - 1);
   const bool act = i < s.n;
-  double x = 0, y = 0, z = 0, q = 0;
+  double x = 0, y = 0, z = 0, q = 0, sx = 0, sy = 0, sz = 0;
   if (act) {
     x = s.x[i];
     y = s.y[i];
     z = s.z[i];
     q = s.q[i];
+    sx = x / s.box[0];
+    sy = y / s.box[1];
+    sz = z / s.box[2];
   }
   double fx = 0, fy = 0, fz = 0;
   double v[6] = {0, 0, 0, 0, 0, 0};
   for (int base = 0; base < kv.P; base += kForceTile) {
     __syncthreads();
-    for (int t = threadIdx.x; t < kForceTile; t += blockDim.x) {
-      int l = base + t;
+    for (int u = threadIdx.x; u < kForceTile; u += blockDim.x) {
+      int l = base + u;
       if (l < kv.P) {
         double kx = kv.kvec[3 * l], ky = kv.kvec[3 * l + 1], kz = kv.kvec[3 * l + 2];
         double k2 = kx * kx + ky * ky + kz * kz;
-        tk[t][0] = kx;
-        tk[t][1] = ky;
-        tk[t][2] = kz;
-        tk[t][3] = ewald ? exp(-k2 / (4 * alpha)) / (s.volume * k2) : 1.0 / (s.volume * k2);
-        tr[t][0] = kv.rho[2 * l];
-        tr[t][1] = kv.rho[2 * l + 1];
+        tk[u][0] = kx;
+        tk[u][1] = ky;
+        tk[u][2] = kz;
+        tk[u][3] = ewald ? exp(-k2 / (4 * alpha)) / (s.volume * k2) : 1.0 / (s.volume * k2);
+        tm[u][0] = k_to_m(kx, s.box[0]);
+        tm[u][1] = k_to_m(ky, s.box[1]);
+        tm[u][2] = k_to_m(kz, s.box[2]);
+        tr[u][0] = kv.rho[2 * l];
+        tr[u][1] = kv.rho[2 * l + 1];
       }
     }
     __syncthreads();
     int cnt = kv.P - base < kForceTile ? kv.P - base : kForceTile;
-    for (int t = 0; t < cnt; ++t) {
-      double kx = tk[t][0], ky = tk[t][1], kz = tk[t][2];
-      double ph = kx * x + ky * y + kz * z;
+    for (int u = sub; u < cnt; u += L) {
+      double kx = tk[u][0], ky = tk[u][1], kz = tk[u][2];
       double sn, cs;
-      sincos(ph, &sn, &cs);
-      double fs = (-4.0 * kPi * q) * tk[t][3] * (cs * tr[t][1] - sn * tr[t][0]);
+      phase_sincos(tm[u][0], tm[u][1], tm[u][2], sx, sy, sz, &sn, &cs);
+      double fs = (-4.0 * kPi * q) * tk[u][3] * (cs * tr[u][1] - sn * tr[u][0]);
       fx += fs * kx;
       fy += fs * ky;
       fz += fs * kz;
@@ -145,23 +167,32 @@ __global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewal
     }
   }
   const double sc = kv.S_over_P;
-  if (act) {
+  if (THERMO) {
+    const double vals[kRowK] = {v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc, v[4] * sc, v[5] * sc};
+    block_write_row<kRowK>(vals, rows + (size_t)blockIdx.x * kRowK);
+  }
+#pragma unroll
+  for (int off = 1; off < L; off <<= 1) {
+    fx += __shfl_xor(fx, off, kWave);
+    fy += __shfl_xor(fy, off, kWave);
+    fz += __shfl_xor(fz, off, kWave);
+    if (PERATOM)
+      for (int k = 0; k < 6; ++k) v[k] += __shfl_xor(v[k], off, kWave);
+  }
+  if (act && sub == 0) {
     s.fkx[i] = s.qqr2e * (fx * sc);
     s.fky[i] = s.qqr2e * (fy * sc);
     s.fkz[i] = s.qqr2e * (fz * sc);
     if (PERATOM)
       for (int k = 0; k < 6; ++k) s.vir_k[(size_t)6 * i + k] = v[k] * sc;
   }
-  if (THERMO) {
-    const double vals[6] = {v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc, v[4] * sc, v[5] * sc};
-    const int slots[6] = {S_VIR_K, S_VIR_K + 1, S_VIR_K + 2, S_VIR_K + 3, S_VIR_K + 4, S_VIR_K + 5};
-    block_write_row<6>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
-  }
 }
+
+int kforce_blocks(const SysView& s) { return (int)((((size_t)s.n << kKLg) + kBlock - 1) / kBlock); }
 
 void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, int thermo, int peratom,
                    double* rows, int* nblocks) {
-  int grid = (s.n + kBlock - 1) / kBlock;
+  int grid = kforce_blocks(s);
   if (nblocks) *nblocks = grid;
   if (peratom)
     hipLaunchKernelGGL((k_kforce<true, true>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);

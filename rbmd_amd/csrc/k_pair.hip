@@ -15,103 +15,23 @@
 //   virial  ComputePairVirial_fix (+1/2, plain) / ComputePairVirial (-1/2, weighted, RBL), :304-332
 #include "dev.hpp"
 #include "reduce.hpp"
+#include "pair.hpp"
 
 namespace rbmd {
 
-__device__ __forceinline__ double gnear(double alpha, double sqrt_alpha, double dis) {
-  return erfc(sqrt_alpha * dis) / (dis * dis) + 2.0 * sqrt_alpha * exp(-alpha * dis * dis) / (sqrt(kPi) * dis);
-}
-
-// step table of the RBL path: G at alpha = 0.1 evaluated at the lower edge of
-// the FP32 bucket given by 3 exponent + 8 mantissa bits (ExecStaticTable.h:38-39,127-135)
-__device__ __forceinline__ double gnear_table(double dis) {
-  float fd = (float)dis;
-  unsigned bits = __float_as_uint(fd) & 0xFFFF8000u;
-  double lo = (double)__uint_as_float(bits);
-  return gnear(0.1, sqrt(0.1), lo);
-}
-
-__device__ __forceinline__ void min_image(const SysView& s, double& rx, double& ry, double& rz) {
-  if (fabs(rx) > s.box[0] * 0.5) rx -= (rx > 0 ? s.box[0] : -s.box[0]);
-  if (fabs(ry) > s.box[1] * 0.5) ry -= (ry > 0 ? s.box[1] : -s.box[1]);
-  if (fabs(rz) > s.box[2] * 0.5) rz -= (rz > 0 ? s.box[2] : -s.box[2]);
-}
-
-__device__ __forceinline__ void decode(const SysView& s, int e, int& j, double& ox, double& oy, double& oz) {
-  j = e >> kShiftBits;
-  int code = e & ((1 << kShiftBits) - 1);
-  ox = s.dxdydz[0] * (double)(((code & 7) - 3) * s.dims[0]);
-  oy = s.dxdydz[1] * (double)((((code >> 3) & 7) - 3) * s.dims[1]);
-  oz = s.dxdydz[2] * (double)((((code >> 6) & 7) - 3) * s.dims[2]);
-}
-
-__device__ __forceinline__ void vir_add(double* v, double rx, double ry, double rz, double fx, double fy, double fz,
-                                        double scale) {
-  fx *= scale;
-  fy *= scale;
-  fz *= scale;
-  v[0] += rx * fx;
-  v[1] += ry * fy;
-  v[2] += rz * fz;
-  v[3] += rx * fy;
-  v[4] += rx * fz;
-  v[5] += ry * fz;
-}
-
-template <int W>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int off = 1; off < W; off <<= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
-
-// One pair term; returns the scalar f such that the force on i is f * r.
-// Reference arithmetic: LJ ExecForceFunction.h:412-435 / 818-841, G(r) :682-689,
-// special weights RunWorklet.cpp:2827-2857.
-template <int MODE, bool EN>
-__device__ __forceinline__ double pair_term(const SysView& s, double rx, double ry, double rz, double d2, double qq,
-                                            double epsij, double sig6, double w, double& elj, double& ec) {
-  const bool in = d2 < s.cut2 && d2 > 0.0001;
-  double flj = 0, fc = 0;
-  elj = 0;
-  ec = 0;
-  if (in) {
-    double d6 = d2 * d2 * d2;
-    double sd6 = sig6 / d6;
-    flj = -24.0 * epsij * ((2.0 * sd6 - 1.0) * sd6) / d2;
-    if (EN) elj = 0.5 * (4.0 * epsij * (sd6 - 1.0) * sd6);
-    if (MODE != 2) {
-      double rinv = rsqrt(d2);
-      double dis = d2 * rinv;
-      double erfcv = erfc(s.sqrt_alpha * dis);
-      double g = erfcv * (rinv * rinv) + (2.0 / sqrt(kPi)) * s.sqrt_alpha * exp(-s.alpha * d2) * rinv;
-      if (MODE == 0) {
-        fc = s.qqr2e * (-qq * g * rinv);
-        if (EN) ec = s.qqr2e * (0.5 * qq * erfcv * rinv);
-      } else {
-        fc = s.qqr2e * (-qq * g * rinv + (1.0 - w) * qq * (rinv * rinv * rinv));
-        if (EN) ec = s.qqr2e * (0.5 * qq * erfcv * rinv - (1.0 - w) * (0.5 * qq * rinv));
-      }
-    }
-  }
-  if (MODE == 1) {
-    flj *= w;
-    elj *= w;
-  }
-  return fc + flj;
-}
-
-constexpr int kPairBatch = 4;
-constexpr int kMaxTypes = 16;
+constexpr int kPairBatch = 2;
 
 // MODE 0: plain LJ+coul, 1: weighted (special bonds), 2: LJ only.
-// kLanes consecutive threads share one atom: lane `sub` takes list slots
-// sub, sub + kLanes, ... of every segment, kPairBatch slots at a time so that
+// L = 2^LG consecutive threads share one atom: lane `sub` takes list slots
+// sub, sub + L, ... of every segment, kPairBatch slots at a time so that
 // several neighbour gathers are in flight per lane, and the partial sums are
-// combined with a fixed xor-shuffle tree at the end (deterministic).
-template <int MODE, bool THERMO, bool PERATOM>
+// combined with a fixed xor-shuffle tree at the end (deterministic).  SP is
+// the register budget for the atom's special list (>= the longest list).
+template <int MODE, bool THERMO, bool PERATOM, int LG, int SP>
 __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows) {
+  constexpr int L = 1 << LG;
   __shared__ double tab_eps[kMaxTypes * kMaxTypes], tab_sig[kMaxTypes * kMaxTypes];
+  __shared__ double ctab[MODE == 2 ? 1 : kCoulM * 2 * (kCoulD + 1)];
   const int nt = s.ntypes;
   const bool lds_tab = nt <= kMaxTypes;
   if (lds_tab)
@@ -119,10 +39,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
       tab_eps[k] = s.eps_ij[k];
       tab_sig[k] = s.sig6_ij[k];
     }
+  if (MODE != 2)
+    for (int k = threadIdx.x; k < kCoulM * 2 * (kCoulD + 1); k += blockDim.x) ctab[k] = s.ctab[k];
   __syncthreads();
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t / kLanes;
-  const int sub = t % kLanes;
+  const int i = t >> LG;
+  const int sub = t & (L - 1);
   constexpr bool EN = THERMO || PERATOM;
   double fx = 0, fy = 0, fz = 0;
   double vir[6] = {0, 0, 0, 0, 0, 0};
@@ -131,19 +53,18 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
     const double4 pi = s.spos[i];
     const int ti = s.stype[i];
     const double qi = pi.w;
-    int spn = 0;
-    int spj[kMaxSpecial];
-    double spw[kMaxSpecial];
+    int spj[SP > 0 ? SP : 1];
+    double spw[SP > 0 ? SP : 1];
     if (MODE == 1) {
       int id = s.point_ids[i];
       int b = s.sp_off[id], e = s.sp_off[id + 1];
-      spn = e - b;
-      if (spn > kMaxSpecial) {
+      int spn = e - b;
+      if (spn > SP) {
         atomicOr(s.status, 4);
-        spn = kMaxSpecial;
+        spn = SP;
       }
 #pragma unroll
-      for (int k = 0; k < kMaxSpecial; ++k) {
+      for (int k = 0; k < SP; ++k) {
         spj[k] = k < spn ? s.rank[s.sp_ids[b + k]] : -1;
         spw[k] = k < spn ? s.sp_w[b + k] : 1.0;
       }
@@ -151,12 +72,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
     for (int seg = 0; seg < s.nseg; ++seg) {
       const int num = s.nbr_num[seg * s.n + i];
       const int base = seg * s.seg_cap;
-      for (int k0 = sub; k0 < num; k0 += kLanes * kPairBatch) {
+      for (int k0 = sub; k0 < num; k0 += L * kPairBatch) {
         int ent[kPairBatch];
 #pragma unroll
         for (int bb = 0; bb < kPairBatch; ++bb) {
-          int k = k0 + bb * kLanes;
-          ent[bb] = k < num ? s.nbr[nbr_index(base + k, i, s.n)] : -1;
+          int k = k0 + bb * L;
+          ent[bb] = k < num ? s.nbr[nbr_index(base + k, i, s.n, LG)] : -1;
         }
         double4 pj[kPairBatch];
         int tj[kPairBatch];
@@ -183,11 +104,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
           double w = 1.0;
           if (MODE == 1) {
 #pragma unroll
-            for (int u = 0; u < kMaxSpecial; ++u)
+            for (int u = 0; u < SP; ++u)
               if (spj[u] == j) w = spw[u];
           }
           double elj, ec;
-          const double sc = pair_term<MODE, EN>(s, rx, ry, rz, d2, qi * pj[bb].w, epsij, sig6, w, elj, ec);
+          const double sc = pair_term<MODE, EN>(s, ctab, d2, qi * pj[bb].w, epsij, sig6, w, elj, ec);
           const double px = sc * rx, py = sc * ry, pz = sc * rz;
           fx += px;
           fy += py;
@@ -202,18 +123,16 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
     }
   }
   if (THERMO) {
-    const double vals[8] = {pelj, pec, vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
-    const int slots[8] = {S_ELJ, S_ECOUL, S_VIR_NEAR, S_VIR_NEAR + 1, S_VIR_NEAR + 2,
-                          S_VIR_NEAR + 3, S_VIR_NEAR + 4, S_VIR_NEAR + 5};
-    block_write_row<8>(vals, slots, rows + (size_t)blockIdx.x * S_NSLOT);
+    const double vals[kRowPair] = {pelj, pec, vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
+    block_write_row<kRowPair>(vals, rows + (size_t)blockIdx.x * kRowPair);
   }
-  fx = group_sum<kLanes>(fx);
-  fy = group_sum<kLanes>(fy);
-  fz = group_sum<kLanes>(fz);
+  fx = group_sum<L>(fx);
+  fy = group_sum<L>(fy);
+  fz = group_sum<L>(fz);
   if (PERATOM) {
-    for (int k = 0; k < 6; ++k) vir[k] = group_sum<kLanes>(vir[k]);
-    pelj = group_sum<kLanes>(pelj);
-    pec = group_sum<kLanes>(pec);
+    for (int k = 0; k < 6; ++k) vir[k] = group_sum<L>(vir[k]);
+    pelj = group_sum<L>(pelj);
+    pec = group_sum<L>(pec);
   }
   if (i < s.n && sub == 0) {
     s.fnx[i] = fx;
@@ -225,6 +144,26 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
       s.e_coul[i] = pec;
     }
   }
+}
+
+template <int MODE, int LG, int SP>
+void launch_pair_v(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int grid) {
+  if (peratom)
+    hipLaunchKernelGGL((k_pair_verlet<MODE, true, true, LG, SP>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+  else if (thermo)
+    hipLaunchKernelGGL((k_pair_verlet<MODE, true, false, LG, SP>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+  else
+    hipLaunchKernelGGL((k_pair_verlet<MODE, false, false, LG, SP>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+}
+
+template <int MODE, int SP>
+void launch_pair_lg(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int grid) {
+  if (s.lanes_lg == 2)
+    launch_pair_v<MODE, 2, SP>(s, st, thermo, peratom, rows, grid);
+  else if (s.lanes_lg == 3)
+    launch_pair_v<MODE, 3, SP>(s, st, thermo, peratom, rows, grid);
+  else
+    launch_pair_v<MODE, 4, SP>(s, st, thermo, peratom, rows, grid);
 }
 
 // RBL: per-atom corrected force before the mean removal; the virial keeps the
@@ -264,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_rbl(SysView s) {
     for (int k = 0; k < num; ++k) {
       int j;
       double ox, oy, oz;
-      decode(s, s.nbr[nbr_index(base + k, i, s.n)], j, ox, oy, oz);
+      decode(s, s.nbr[nbr_index(base + k, i, s.n, s.lanes_lg)], j, ox, oy, oz);
       const double4 pj = s.spos[j];
       double rx = (pj.x - ox) - pi.x;
       double ry = (pj.y - oy) - pi.y;
@@ -361,13 +300,15 @@ __global__ __launch_bounds__(1024) void k_rbl_mean(SysView s) {
   }
 }
 
+int pair_blocks(const SysView& s) {
+  if (s.rbl && !s.lj_only) return (s.n + kBlock - 1) / kBlock;
+  return (int)(((size_t)s.n << s.lanes_lg) + kBlock - 1) / kBlock;
+}
+
 void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int* nblocks) {
-  int grid_rbl = (s.n + kBlock - 1) / kBlock;
-  int grid = (s.n * kLanes + kBlock - 1) / kBlock;
+  const int grid = pair_blocks(s);
   if (nblocks) *nblocks = grid;
   if (s.rbl && !s.lj_only) {
-    grid = grid_rbl;
-    if (nblocks) *nblocks = grid;
     if (s.weighted)
       hipLaunchKernelGGL(k_pair_rbl<true>, dim3(grid), dim3(kBlock), 0, st, s);
     else
@@ -375,22 +316,14 @@ void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, doub
     hipLaunchKernelGGL(k_rbl_mean, dim3(1), dim3(1024), 0, st, s);
     return;
   }
-  const int mode = s.lj_only ? 2 : (s.weighted ? 1 : 0);
-#define RBMD_PAIR(M)                                                                                  \
-  if (peratom)                                                                                        \
-    hipLaunchKernelGGL((k_pair_verlet<M, true, true>), dim3(grid), dim3(kBlock), 0, st, s, rows);     \
-  else if (thermo)                                                                                    \
-    hipLaunchKernelGGL((k_pair_verlet<M, true, false>), dim3(grid), dim3(kBlock), 0, st, s, rows);    \
-  else                                                                                                \
-    hipLaunchKernelGGL((k_pair_verlet<M, false, false>), dim3(grid), dim3(kBlock), 0, st, s, rows);
-  if (mode == 0) {
-    RBMD_PAIR(0)
-  } else if (mode == 1) {
-    RBMD_PAIR(1)
-  } else {
-    RBMD_PAIR(2)
-  }
-#undef RBMD_PAIR
+  if (s.lj_only)
+    launch_pair_lg<2, 0>(s, st, thermo, peratom, rows, grid);
+  else if (!s.weighted)
+    launch_pair_lg<0, 0>(s, st, thermo, peratom, rows, grid);
+  else if (s.sp_max <= 4)
+    launch_pair_lg<1, 4>(s, st, thermo, peratom, rows, grid);
+  else
+    launch_pair_lg<1, 16>(s, st, thermo, peratom, rows, grid);
 }
 
 }  // namespace rbmd

@@ -1,0 +1,105 @@
+// pair.hpp -- the real-space pair term (LJ + erfc Coulomb) shared by the
+// neighbour-list pair kernels and the fused cell-stencil force kernel.
+#pragma once
+#include "dev.hpp"
+
+namespace rbmd {
+
+__device__ __forceinline__ double gnear(double alpha, double sqrt_alpha, double dis) {
+  return erfc(sqrt_alpha * dis) / (dis * dis) + 2.0 * sqrt_alpha * exp(-alpha * dis * dis) / (sqrt(kPi) * dis);
+}
+
+// step table of the RBL path: G at alpha = 0.1 evaluated at the lower edge of
+// the FP32 bucket given by 3 exponent + 8 mantissa bits (ExecStaticTable.h:38-39,127-135)
+__device__ __forceinline__ double gnear_table(double dis) {
+  float fd = (float)dis;
+  unsigned bits = __float_as_uint(fd) & 0xFFFF8000u;
+  double lo = (double)__uint_as_float(bits);
+  return gnear(0.1, sqrt(0.1), lo);
+}
+
+__device__ __forceinline__ void min_image(const SysView& s, double& rx, double& ry, double& rz) {
+  if (fabs(rx) > s.box[0] * 0.5) rx -= (rx > 0 ? s.box[0] : -s.box[0]);
+  if (fabs(ry) > s.box[1] * 0.5) ry -= (ry > 0 ? s.box[1] : -s.box[1]);
+  if (fabs(rz) > s.box[2] * 0.5) rz -= (rz > 0 ? s.box[2] : -s.box[2]);
+}
+
+__device__ __forceinline__ void decode(const SysView& s, int e, int& j, double& ox, double& oy, double& oz) {
+  j = e >> kShiftBits;
+  int code = e & ((1 << kShiftBits) - 1);
+  ox = s.dxdydz[0] * (double)(((code & 7) - 3) * s.dims[0]);
+  oy = s.dxdydz[1] * (double)((((code >> 3) & 7) - 3) * s.dims[1]);
+  oz = s.dxdydz[2] * (double)((((code >> 6) & 7) - 3) * s.dims[2]);
+}
+
+__device__ __forceinline__ void vir_add(double* v, double rx, double ry, double rz, double fx, double fy, double fz,
+                                        double scale) {
+  fx *= scale;
+  fy *= scale;
+  fz *= scale;
+  v[0] += rx * fx;
+  v[1] += ry * fy;
+  v[2] += rz * fz;
+  v[3] += rx * fy;
+  v[4] += rx * fz;
+  v[5] += ry * fz;
+}
+
+template <int W>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = 1; off < W; off <<= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// One pair term; returns the scalar f such that the force on i is f * r.
+// Reference arithmetic: LJ ExecForceFunction.h:412-435 / 818-841, G(r) :682-689,
+// special weights RunWorklet.cpp:2827-2857.  With rinv = 1/r:
+//   LJ     f = -24 eps (2 s6 - 1) s6 rinv^2,    s6 = sigma^6 rinv^6
+//   coul   f = -qq G(r)/r = -qq P(r) rinv^3,     e = qq Q(r) rinv / 2
+//   weight f += (1 - w) qq rinv^3,               e -= (1 - w) qq rinv / 2
+// P and Q come from the LDS copy of the piecewise polynomial table.
+template <int MODE, bool EN>
+__device__ __forceinline__ double pair_term(const SysView& s, const double* ctab, double d2, double qq, double epsij,
+                                            double sig6, double w, double& elj, double& ec) {
+  const bool in = d2 < s.cut2 && d2 > 0.0001;
+  double flj = 0, fc = 0;
+  elj = 0;
+  ec = 0;
+  if (in) {
+    const double rinv = rsqrt(d2);
+    const double rinv2 = rinv * rinv;
+    const double sd6 = sig6 * (rinv2 * rinv2 * rinv2);
+    flj = -24.0 * epsij * ((2.0 * sd6 - 1.0) * sd6) * rinv2;
+    if (EN) elj = 0.5 * (4.0 * epsij * (sd6 - 1.0) * sd6);
+    if (MODE != 2) {
+      const double fk = (d2 * rinv) * s.ctab_inv_h;
+      int k = (int)fk;
+      k = k < kCoulM - 1 ? k : kCoulM - 1;
+      const double u = fk - (double)k - 0.5;
+      const double* c = ctab + k * 2 * (kCoulD + 1);
+      double P = c[kCoulD];
+#pragma unroll
+      for (int d = kCoulD - 1; d >= 0; --d) P = fma(P, u, c[d]);
+      const double rinv3 = rinv2 * rinv;
+      if (MODE == 0) {
+        fc = s.qqr2e * (-qq * P * rinv3);
+      } else {
+        fc = s.qqr2e * (qq * rinv3 * ((1.0 - w) - P));
+      }
+      if (EN) {
+        double Q = c[2 * kCoulD + 1];
+#pragma unroll
+        for (int d = kCoulD - 1; d >= 0; --d) Q = fma(Q, u, c[kCoulD + 1 + d]);
+        ec = MODE == 0 ? s.qqr2e * (0.5 * qq * Q * rinv) : s.qqr2e * (0.5 * qq * rinv * (Q - (1.0 - w)));
+      }
+    }
+  }
+  if (MODE == 1) {
+    flj *= w;
+    elj *= w;
+  }
+  return fc + flj;
+}
+
+}  // namespace rbmd

@@ -10,11 +10,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum NV values over the workgroup (kBlock threads) and write one row of
-// S_NSLOT doubles: the NV sums go to `slots`, every other slot gets zero.
-template <int NV>
-__device__ __forceinline__ void block_write_row(const double (&vals)[NV], const int (&slots)[NV], double* row) {
-  __shared__ double lds[kBlock / kWave][NV > 0 ? NV : 1];
+// Sum NV values over the workgroup (NW wavefronts) and write them as one row
+// of NV doubles (fixed-order sums: wavefront shuffle tree, then wavefronts in
+// order).  Every thread of the workgroup must call this.
+template <int NV, int NW = kBlock / kWave>
+__device__ __forceinline__ void block_write_row(const double (&vals)[NV], double* row) {
+  __shared__ double lds[NW][NV];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
 #pragma unroll
@@ -23,16 +24,10 @@ __device__ __forceinline__ void block_write_row(const double (&vals)[NV], const 
     if (lane == 0) lds[wave][k] = t;
   }
   __syncthreads();
-  for (int s = threadIdx.x; s < S_NSLOT; s += blockDim.x) {
-    double out = 0.0;
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      if (slots[k] == s) {
-        double acc = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) acc += lds[w][k];
-        out = acc;
-      }
-    row[s] = out;
+  if ((int)threadIdx.x < NV) {
+    double acc = 0.0;
+    for (int w = 0; w < NW; ++w) acc += lds[w][threadIdx.x];
+    row[threadIdx.x] = acc;
   }
 }
 

@@ -232,6 +232,9 @@ class GpuSystem:
         check(self.L.rbmd_neighbors_export(self.h, int(cap), _p(ids, C.c_int32), _p(off), _p(num, C.c_int32)))
         return ids.reshape(self.n, cap), off.reshape(self.n, cap, 3), num
 
+    def build_neighbors(self, cap=0):
+        check(self.L.rbmd_neighbors_build(self.h, int(cap)))
+
     def neighbor_entries(self):
         t = C.c_int64()
         check(self.L.rbmd_neighbors_count(self.h, C.byref(t)))

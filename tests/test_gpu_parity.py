@@ -14,10 +14,12 @@ from tests.systems import rel_err, salt, salt_run, water, water_run
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-10
+PATHS = ["cells", "lists"]  # fused cell-stencil kernel / stored Verlet lists + pair kernel
 
 
 def pair(ctx, system, run):
-    print(f"[case] n={system.n_atoms} kind={system.kind} nbr={run.neighbor_type} ff={run.force_field}", flush=True)
+    print(f"[case] n={system.n_atoms} kind={system.kind} nbr={run.neighbor_type} ff={run.force_field} "
+          f"path={run.pair_path}", flush=True)
     return GpuSystem(ctx, system, run), oracle.OracleSystem(system, run)
 
 
@@ -68,9 +70,10 @@ def test_neighbor_counts_bruteforce(ctx):
     assert np.array_equal(gn, o.bruteforce_counts(4.0))
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("kind", ["salt", "water"])
-def test_real_space_force(ctx, kind):
-    s, r = (salt(), salt_run()) if kind == "salt" else (water(), water_run())
+def test_real_space_force(ctx, kind, path):
+    s, r = (salt(), salt_run(pair_path=path)) if kind == "salt" else (water(), water_run(pair_path=path))
     g, o = pair(ctx, s, r)
     g.neighbors(0)
     o.neighbors(0)
@@ -82,9 +85,10 @@ def test_real_space_force(ctx, kind):
     assert abs(gc.sum() - oc.sum()) <= FP64_TOL * abs(oc).sum()
 
 
-def test_lj_only_force(ctx):
+@pytest.mark.parametrize("path", PATHS)
+def test_lj_only_force(ctx, path):
     s = salt(n=2000, seed=9)
-    r = salt_run(force_field="LJ/CUT", coulomb_type="NONE", cut_off=2.5, r_core=2.5)
+    r = salt_run(force_field="LJ/CUT", coulomb_type="NONE", cut_off=2.5, r_core=2.5, pair_path=path)
     g, o = pair(ctx, s, r)
     g.neighbors(0)
     o.neighbors(0)
@@ -183,19 +187,22 @@ def _md_compare(ctx, system, run, steps, tol):
     return tg
 
 
-def test_md_salt_nvt(ctx):
-    _md_compare(ctx, salt(), salt_run(), 20, 1e-8)
+@pytest.mark.parametrize("path", PATHS)
+def test_md_salt_nvt(ctx, path):
+    _md_compare(ctx, salt(), salt_run(pair_path=path), 20, 1e-8)
 
 
-def test_md_water_nvt_shake(ctx):
-    t = _md_compare(ctx, water(), water_run(), 20, 1e-8)
+@pytest.mark.parametrize("path", PATHS)
+def test_md_water_nvt_shake(ctx, path):
+    t = _md_compare(ctx, water(), water_run(pair_path=path), 20, 1e-8)
     assert t["status_flags"] == 0
 
 
-def test_md_water_misfit_path(ctx):
+@pytest.mark.parametrize("path", PATHS)
+def test_md_water_misfit_path(ctx, path):
     # SHAKE moves atoms after binning; over 60 steps some cross a cell face and
     # take the per-atom stencil path; the trajectory must still match
-    g, o = pair(ctx, water(), water_run())
+    g, o = pair(ctx, water(), water_run(pair_path=path))
     g.md_init()
     o.md_init()
     seen = 0
