@@ -254,43 +254,49 @@ void rbl_capacity(const rbmd_params& p, int n, double volume, int* rs_num, int* 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // Piecewise polynomial table of the Coulomb radial factors over [0, rc]:
-// interval k = [k h, (k+1) h], local variable u = r/h - k - 1/2 in [-1/2, 1/2],
-// degree-kCoulD interpolation at Chebyshev nodes, solved in long double.
+// interval k = [k h, (k+1) h], local variable u = r/h - k - 1/2 in [-1/2, 1/2].
+// Per interval the degree-kCoulD Chebyshev interpolant at the Chebyshev nodes
+// (coefficients by the discrete cosine sums, then converted to powers of u),
+// all in long double.
 // P(r) = erfc(beta r) + 2 beta/sqrt(pi) r exp(-beta^2 r^2),  Q(r) = erfc(beta r).
+// Layout (doubles): [f][d / 2][k][d % 2], f = 0 P, 1 Q.
 std::vector<double> coul_table(double beta, double rc) {
-  const int M = kCoulM, D = kCoulD, K = D + 1;
+  const int M = kCoulM, K = kCoulD + 1;
+  const long double pi = 3.14159265358979323846264338327950288L;
   const long double h = (long double)rc / M;
   const long double two_over_sqrtpi = 1.1283791670955125738961589031215452L;
-  std::vector<double> tab((size_t)M * 2 * K);
-  long double u[K];
-  for (int j = 0; j < K; ++j) u[j] = 0.5L * cosl(3.14159265358979323846264338327950288L * (j + 0.5L) / K);
+  std::vector<double> tab((size_t)2 * K * M);
+  // monomial coefficients of T_n(x): tcoef[n][d]
+  std::vector<std::vector<long double>> tcoef(K, std::vector<long double>(K, 0.0L));
+  tcoef[0][0] = 1;
+  if (K > 1) tcoef[1][1] = 1;
+  for (int n = 2; n < K; ++n)
+    for (int d = 0; d < K; ++d)
+      tcoef[n][d] = (d > 0 ? 2 * tcoef[n - 1][d - 1] : 0.0L) - tcoef[n - 2][d];
   for (int k = 0; k < M; ++k) {
     for (int f = 0; f < 2; ++f) {
-      long double A[K][K + 1];
+      long double y[K], x[K];
       for (int j = 0; j < K; ++j) {
-        long double r = (k + 0.5L + u[j]) * h;
-        long double br = (long double)beta * r;
-        long double y = erfcl(br);
-        if (f == 0) y += two_over_sqrtpi * (long double)beta * r * expl(-br * br);
-        long double pw = 1;
-        for (int d = 0; d < K; ++d) {
-          A[j][d] = pw;
-          pw *= u[j];
-        }
-        A[j][K] = y;
+        x[j] = cosl(pi * (j + 0.5L) / K);  // node in [-1, 1]; u = x / 2
+        const long double r = (k + 0.5L + 0.5L * x[j]) * h;
+        const long double br = (long double)beta * r;
+        y[j] = erfcl(br);
+        if (f == 0) y[j] += two_over_sqrtpi * (long double)beta * r * expl(-br * br);
       }
-      for (int col = 0; col < K; ++col) {  // Gaussian elimination with partial pivoting
-        int piv = col;
-        for (int rr = col + 1; rr < K; ++rr)
-          if (fabsl(A[rr][col]) > fabsl(A[piv][col])) piv = rr;
-        for (int cc = 0; cc <= K; ++cc) std::swap(A[col][cc], A[piv][cc]);
-        for (int rr = 0; rr < K; ++rr) {
-          if (rr == col) continue;
-          long double fct = A[rr][col] / A[col][col];
-          for (int cc = col; cc <= K; ++cc) A[rr][cc] -= fct * A[col][cc];
-        }
+      long double mono[K];
+      for (int d = 0; d < K; ++d) mono[d] = 0;
+      for (int nn = 0; nn < K; ++nn) {
+        long double a = 0;
+        for (int j = 0; j < K; ++j) a += y[j] * cosl(pi * nn * (j + 0.5L) / K);
+        a *= 2.0L / K;
+        if (nn == 0) a *= 0.5L;
+        for (int d = 0; d <= nn; ++d) mono[d] += a * tcoef[nn][d];  // in x = 2u
       }
-      for (int d = 0; d < K; ++d) tab[((size_t)k * 2 + f) * K + d] = (double)(A[d][K] / A[d][d]);
+      long double scale = 1;
+      for (int d = 0; d < K; ++d) {  // x^d = 2^d u^d
+        tab[(((size_t)f * (K / 2) + d / 2) * M + k) * 2 + (d % 2)] = (double)(mono[d] * scale);
+        scale *= 2;
+      }
     }
   }
   return tab;
@@ -744,7 +750,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     std::vector<double> ct = coul_table(v.sqrt_alpha > 0 ? v.sqrt_alpha : 0.0, p->cut_off);
     double* dct = nullptr;
     A(s->upload(&dct, ct.data(), ct.size()));
-    v.ctab = dct;
+    v.ctab = reinterpret_cast<const double2*>(dct);
     v.ctab_inv_h = kCoulM / p->cut_off;
   }
   if (have_sp) {
@@ -1124,7 +1130,7 @@ rbmd_status rbmd_neighbors_export(rbmd_sys* s, int32_t cap, int32_t* ids, double
         over = true;
         return;
       }
-      int e = nbr[nbr_index(slot, i, n, s->v.lanes_lg)];
+      int e = nbr[nbr_index(slot, i, s->v.cap)];
       int js = e >> kShiftBits;
       int code = e & ((1 << kShiftBits) - 1);
       int sh[3] = {(code & 7) - 3, ((code >> 3) & 7) - 3, ((code >> 6) & 7) - 3};

@@ -13,16 +13,15 @@ constexpr int kChunk = 64;      // i-atoms per neighbour-build work item (one wa
 constexpr int kMaxSpecial = 8;  // special-list entries held in registers per atom (RBL kernels)
 constexpr int kShiftBits = 9;   // 3 bits per axis image shift in packed neighbour entries
 constexpr int kMaxLanes = 16;   // most lanes sharing one atom's neighbour list
-constexpr int kCoulM = 64;      // Coulomb table intervals over [0, rc]
-constexpr int kCoulD = 7;       // polynomial degree per interval
+constexpr int kCoulM = 16;      // Coulomb table intervals over [0, rc]
+constexpr int kCoulD = 11;      // polynomial degree per interval
+constexpr int kCoulPairs = (kCoulD + 1) / 2;           // coefficient pairs per interval
+constexpr int kCoulTab = 2 * kCoulPairs * kCoulM;      // double2 entries (P and Q)
 constexpr int kMaxTypes = 16;   // type-pair tables held in LDS up to this many types
 
-// Neighbour-list slot layout for L = 2^lg lanes per atom: slot k of atom i
-// (sorted order) lives at ((k >> lg) * n + i) << lg | (k & (L-1)), so the L
-// lanes that share atom i, and consecutive atoms, read consecutive words.
-__host__ __device__ __forceinline__ size_t nbr_index(int slot, int i, int n, int lg) {
-  return ((((size_t)(slot >> lg)) * n + i) << lg) | (size_t)(slot & ((1 << lg) - 1));
-}
+// Neighbour-list slot layout: one row of `cap` slots per atom (sorted order),
+// so a row's consecutive entries are consecutive words.
+__host__ __device__ __forceinline__ size_t nbr_index(int slot, int i, int cap) { return (size_t)i * cap + slot; }
 
 // Everything a kernel needs to know about one system.  Passed by value.
 struct SysView {
@@ -54,10 +53,12 @@ struct SysView {
   // per type pair tables [ntypes*ntypes]
   double *eps_ij, *sig6_ij;
   // Coulomb radial table: kCoulM intervals of width h over [0, rc]; per
-  // interval the degree-7 coefficients (in u = r/h - k - 1/2) of
+  // interval the degree-kCoulD coefficients (in u = r/h - k - 1/2) of
   // P(r) = erfc(beta r) + 2 beta/sqrt(pi) r e^{-beta^2 r^2} (so G(r)/r = P / r^3)
-  // and of Q(r) = erfc(beta r)
-  const double* ctab;  // [kCoulM][2][kCoulD + 1]
+  // and of Q(r) = erfc(beta r), coefficient-major: ctab[(f * kCoulPairs + d/2) *
+  // kCoulM + k] holds coefficients d, d+1 (d even) of function f (0 P, 1 Q) on
+  // interval k, so lanes at different distances read different LDS banks
+  const double2* ctab;  // [2][kCoulPairs][kCoulM]
   double ctab_inv_h;
   // special lists (CSR) and molecule groups
   int *sp_off, *sp_ids;
@@ -78,7 +79,7 @@ struct SysView {
   double4* spos;    // [n] x,y,z,q in sorted order
   int* stype;       // [n]
   // neighbour lists, column-major [cap][n] of packed (j_sorted << 9 | shift code)
-  int* nbr;         // layout nbr_index(slot, i, n)
+  int* nbr;         // layout nbr_index(slot, i, cap): one row per atom
   int lds_bins;     // binning kernels aggregate counts in LDS (ncell small enough)
   int* nbr_num;     // VERLET [nseg][n] counts per segment; RBL [2n] (core, shell kept)
   // per-atom force components (original order)

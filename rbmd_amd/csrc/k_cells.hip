@@ -261,7 +261,7 @@ __device__ void misfit_block(const SysView& s, int wave, int lane) {
       if (ok) {
         int slot = cnt + __popcll(bal & ((1ull << lane) - 1ull));
         if (slot < s.seg_cap)
-          s.nbr[nbr_index(seg0 + slot, i, s.n, s.lanes_lg)] = entry;
+          s.nbr[nbr_index(seg0 + slot, i, s.cap)] = entry;
         else
           atomicOr(s.status, 4);
       }
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_nbr_build_verlet(SysView 
       const int u = __builtin_ctzll(acc);
       acc &= acc - 1;
       if (cnt < s.seg_cap)
-        s.nbr[nbr_index(seg0 + cnt, i, s.n, s.lanes_lg)] = stage_e[wave][u];
+        s.nbr[nbr_index(seg0 + cnt, i, s.cap)] = stage_e[wave][u];
       else
         atomicOr(s.status, 4);
       cnt++;
@@ -381,7 +381,7 @@ __device__ __forceinline__ void rbl_consider(const SysView& s, RblState& L, int 
     if (L.shell_seen % s.pice_num == 0) {
       int slot = s.rs_num + L.shell_kept;
       if (slot < s.cap)
-        s.nbr[nbr_index(slot, i, s.n, s.lanes_lg)] = entry;
+        s.nbr[nbr_index(slot, i, s.cap)] = entry;
       else
         atomicOr(s.status, 4);
       L.shell_kept++;
@@ -389,7 +389,7 @@ __device__ __forceinline__ void rbl_consider(const SysView& s, RblState& L, int 
     L.shell_seen++;
   } else if (s.rs2 - d2 > 0.0001 && d2 > 0.0001) {
     if (L.core < s.rs_num)
-      s.nbr[nbr_index(L.core, i, s.n, s.lanes_lg)] = entry;
+      s.nbr[nbr_index(L.core, i, s.cap)] = entry;
     else
       atomicOr(s.status, 4);
     L.core++;
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_misfit_rbl(SysView s) {
             if (is_core) {
               int slot = core + __popcll(bc & below);
               if (slot < s.rs_num)
-                s.nbr[nbr_index(slot, i, s.n, s.lanes_lg)] = entry;
+                s.nbr[nbr_index(slot, i, s.cap)] = entry;
               else
                 atomicOr(s.status, 4);
             }
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_misfit_rbl(SysView s) {
             if (keep) {
               int slot = s.rs_num + kept + __popcll(bk & below);
               if (slot < s.cap)
-                s.nbr[nbr_index(slot, i, s.n, s.lanes_lg)] = entry;
+                s.nbr[nbr_index(slot, i, s.cap)] = entry;
               else
                 atomicOr(s.status, 4);
             }

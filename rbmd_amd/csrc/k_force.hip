@@ -96,7 +96,7 @@ __device__ __forceinline__ void pair_vector(const SysView& s, double& rx, double
 // a = -1, 0, 1; lanes are candidates.  Totals: lanes (shuffle tree), then the
 // wavefronts in order through `buf` (>= kSlabWaves * kWave doubles).
 template <int MODE, bool THERMO, bool PERATOM, int SP>
-__device__ void misfit_force(const SysView& s, const double* ctab, const double* teps, const double* tsig, int wave,
+__device__ void misfit_force(const SysView& s, const double2* ctab, const double* teps, const double* tsig, int wave,
                              int lane, double* buf, ThermoAcc& tacc) {
   constexpr bool EN = THERMO || PERATOM;
   constexpr int NV = PERATOM ? 11 : 3;
@@ -201,14 +201,14 @@ __global__ __launch_bounds__(kSlabThreads) void k_force_cells(SysView s, double*
   __shared__ double facc[kSlabWaves][NACC][kWave];
   // tables
   __shared__ double teps[kMaxTypes * kMaxTypes], tsig[kMaxTypes * kMaxTypes];
-  __shared__ double ctab[MODE == 2 ? 1 : kCoulM * 2 * (kCoulD + 1)];
+  __shared__ double2 ctab[MODE == 2 ? 1 : kCoulTab];
   const int nt = s.ntypes;
   for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
     teps[k] = s.eps_ij[k];
     tsig[k] = s.sig6_ij[k];
   }
   if (MODE != 2)
-    for (int k = threadIdx.x; k < kCoulM * 2 * (kCoulD + 1); k += blockDim.x) ctab[k] = s.ctab[k];
+    for (int k = threadIdx.x; k < kCoulTab; k += blockDim.x) ctab[k] = s.ctab[k];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
   ThermoAcc tacc;

@@ -1,8 +1,10 @@
 // k_pair.hip -- real-space LJ + erfc pair forces over the neighbour lists.
 //
-// One thread per atom in cell-sorted order; each thread walks its own list
-// (column-major, coalesced across the wavefront) and gathers neighbour
-// positions from the sorted double4 array, which stays resident in L2.
+// L consecutive threads share one atom (cell-sorted order); each lane walks the
+// atom's list row, whose 9 column segments it treats as one concatenated list
+// (lane `sub` takes positions sub, sub + L, ...), gathers neighbour positions
+// from the sorted double4 array (resident in L2) and keeps its partial sums in
+// registers; the L partials are combined with a fixed xor-shuffle tree.
 // Forces are gathered, never scattered: no atomics, deterministic sums.
 //
 // Formulas (all FP64 here):
@@ -19,120 +21,170 @@
 
 namespace rbmd {
 
-constexpr int kPairBatch = 2;
+constexpr int kMaxSeg = 9;  // VERLET list segments per atom (stencil columns)
 
-// MODE 0: plain LJ+coul, 1: weighted (special bonds), 2: LJ only.
-// L = 2^LG consecutive threads share one atom: lane `sub` takes list slots
-// sub, sub + L, ... of every segment, kPairBatch slots at a time so that
-// several neighbour gathers are in flight per lane, and the partial sums are
-// combined with a fixed xor-shuffle tree at the end (deterministic).  SP is
-// the register budget for the atom's special list (>= the longest list).
-template <int MODE, bool THERMO, bool PERATOM, int LG, int SP>
+__device__ __forceinline__ int pick_seg(const int (&c)[kMaxSeg], int k) {
+  int r = c[0];
+#pragma unroll
+  for (int t = 1; t < kMaxSeg; ++t) r = k == t ? c[t] : r;
+  return r;
+}
+
+// A lane's position in an atom's concatenated segments: `seg` (nseg past the
+// end), the offset in it, its length (INT_MAX past the end) and its start.
+struct ListCursor {
+  int seg, off, num;
+  const int* p;
+};
+
+// entry at the cursor (-1 past the end), then one stride forward; crossing
+// into the next segment is the rare path
+__device__ __forceinline__ int cursor_take(ListCursor& c, const int (&cnt)[kMaxSeg], int nseg, int seg_cap,
+                                           int stride) {
+  if (__builtin_expect(c.off >= c.num, 0)) {
+    do {
+      c.off -= c.num;
+      ++c.seg;
+      c.p += seg_cap;
+      c.num = c.seg < nseg ? pick_seg(cnt, c.seg) : 0x7fffffff;
+    } while (c.off >= c.num);
+  }
+  const int e = c.seg < nseg ? c.p[c.off] : -1;
+  c.off += stride;
+  return e;
+}
+
+// MODE 0: plain LJ+coul, 1: weighted (special bonds), 2: LJ only.  SP: the
+// register budget for the atom's special list (>= the longest list).  LTAB:
+// type-pair tables in LDS (ntypes <= kMaxTypes).  Two entries per lane per
+// iteration; the neighbour positions of the next iteration are gathered
+// while the current one is evaluated, and its entries one iteration earlier.
+template <int MODE, bool THERMO, bool PERATOM, int SP, bool LTAB>
 __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows) {
-  constexpr int L = 1 << LG;
-  __shared__ double tab_eps[kMaxTypes * kMaxTypes], tab_sig[kMaxTypes * kMaxTypes];
-  __shared__ double ctab[MODE == 2 ? 1 : kCoulM * 2 * (kCoulD + 1)];
+  constexpr bool EN = THERMO || PERATOM;
+  constexpr int SPD = SP > 0 ? SP : 1;
+  __shared__ double teps[LTAB ? kMaxTypes * kMaxTypes : 1], tsig[LTAB ? kMaxTypes * kMaxTypes : 1];
+  __shared__ double2 ctab[MODE == 2 ? 1 : kCoulTab];
   const int nt = s.ntypes;
-  const bool lds_tab = nt <= kMaxTypes;
-  if (lds_tab)
+  if (LTAB)
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
-      tab_eps[k] = s.eps_ij[k];
-      tab_sig[k] = s.sig6_ij[k];
+      teps[k] = s.eps_ij[k];
+      tsig[k] = s.sig6_ij[k];
     }
   if (MODE != 2)
-    for (int k = threadIdx.x; k < kCoulM * 2 * (kCoulD + 1); k += blockDim.x) ctab[k] = s.ctab[k];
+    for (int k = threadIdx.x; k < kCoulTab; k += blockDim.x) ctab[k] = s.ctab[k];
   __syncthreads();
+  const int lg = s.lanes_lg;
+  const int L = 1 << lg;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t >> LG;
+  const int i = t >> lg;
   const int sub = t & (L - 1);
-  constexpr bool EN = THERMO || PERATOM;
   double fx = 0, fy = 0, fz = 0;
   double vir[6] = {0, 0, 0, 0, 0, 0};
   double pelj = 0, pec = 0;
   if (i < s.n) {
     const double4 pi = s.spos[i];
     const int ti = s.stype[i];
-    const double qi = pi.w;
-    int spj[SP > 0 ? SP : 1];
-    double spw[SP > 0 ? SP : 1];
+    int spj[SPD];
+    double spw[SPD];
     if (MODE == 1) {
-      int id = s.point_ids[i];
-      int b = s.sp_off[id], e = s.sp_off[id + 1];
-      int spn = e - b;
+      const int id = s.point_ids[i];
+      const int b = s.sp_off[id];
+      int spn = s.sp_off[id + 1] - b;
       if (spn > SP) {
         atomicOr(s.status, 4);
         spn = SP;
       }
 #pragma unroll
-      for (int k = 0; k < SP; ++k) {
+      for (int k = 0; k < SPD; ++k) {
         spj[k] = k < spn ? s.rank[s.sp_ids[b + k]] : -1;
         spw[k] = k < spn ? s.sp_w[b + k] : 1.0;
       }
     }
-    for (int seg = 0; seg < s.nseg; ++seg) {
-      const int num = s.nbr_num[seg * s.n + i];
-      const int base = seg * s.seg_cap;
-      for (int k0 = sub; k0 < num; k0 += L * kPairBatch) {
-        int ent[kPairBatch];
+    int cnt[kMaxSeg];
 #pragma unroll
-        for (int bb = 0; bb < kPairBatch; ++bb) {
-          int k = k0 + bb * L;
-          ent[bb] = k < num ? s.nbr[nbr_index(base + k, i, s.n, LG)] : -1;
+    for (int g = 0; g < kMaxSeg; ++g) cnt[g] = g < s.nseg ? s.nbr_num[g * s.n + i] : 0;
+    const int nseg = s.nseg, seg_cap = s.seg_cap;
+    ListCursor cur{0, sub, cnt[0], s.nbr + (size_t)i * s.cap};
+    // pipeline: entries two iterations ahead, positions one iteration ahead
+    int ea = cursor_take(cur, cnt, nseg, seg_cap, L);
+    int eb = cursor_take(cur, cnt, nseg, seg_cap, L);
+    // neighbour index of an entry (the atom itself for -1: a harmless load)
+    auto jof = [i](int e) -> unsigned { return e >= 0 ? (unsigned)e >> kShiftBits : (unsigned)i; };
+    double4 pa = s.spos[jof(ea)];
+    double4 pb = s.spos[jof(eb)];
+    int ta = s.stype[jof(ea)];
+    int tb = s.stype[jof(eb)];
+    int na = cursor_take(cur, cnt, nseg, seg_cap, L);
+    int nb = cursor_take(cur, cnt, nseg, seg_cap, L);
+    while (ea >= 0) {
+      const double4 qa = s.spos[jof(na)];
+      const double4 qb = s.spos[jof(nb)];
+      const int ua = s.stype[jof(na)];
+      const int ub = s.stype[jof(nb)];
+      const int ma = cursor_take(cur, cnt, nseg, seg_cap, L);
+      const int mb = cursor_take(cur, cnt, nseg, seg_cap, L);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = h == 0 ? ea : eb;
+        if (e < 0) continue;
+        const double4 p = h == 0 ? pa : pb;
+        const int tj = h == 0 ? ta : tb;
+        int j;
+        double ox, oy, oz;
+        decode(s, e, j, ox, oy, oz);  // the build's image offset, bit for bit
+        double rx = (p.x - ox) - pi.x;
+        double ry = (p.y - oy) - pi.y;
+        double rz = (p.z - oz) - pi.z;
+        if (s.use_min_image) min_image(s, rx, ry, rz);
+        const double d2 = rx * rx + ry * ry + rz * rz;
+        const int tix = ti * nt + tj;
+        const double epsij = LTAB ? teps[tix] : s.eps_ij[tix];
+        const double sig6 = LTAB ? tsig[tix] : s.sig6_ij[tix];
+        double w = 1.0;
+        if (MODE == 1) {
+#pragma unroll
+          for (int u = 0; u < SPD; ++u)
+            if (spj[u] == j) w = spw[u];
         }
-        double4 pj[kPairBatch];
-        int tj[kPairBatch];
-#pragma unroll
-        for (int bb = 0; bb < kPairBatch; ++bb) {
-          int j = ent[bb] >= 0 ? (ent[bb] >> kShiftBits) : i;
-          pj[bb] = s.spos[j];
-          tj[bb] = s.stype[j];
-        }
-#pragma unroll
-        for (int bb = 0; bb < kPairBatch; ++bb) {
-          if (ent[bb] < 0) continue;
-          int j;
-          double ox, oy, oz;
-          decode(s, ent[bb], j, ox, oy, oz);
-          double rx = (pj[bb].x - ox) - pi.x;
-          double ry = (pj[bb].y - oy) - pi.y;
-          double rz = (pj[bb].z - oz) - pi.z;
-          min_image(s, rx, ry, rz);
-          const double d2 = rx * rx + ry * ry + rz * rz;
-          const int tix = ti * nt + tj[bb];
-          const double epsij = lds_tab ? tab_eps[tix] : s.eps_ij[tix];
-          const double sig6 = lds_tab ? tab_sig[tix] : s.sig6_ij[tix];
-          double w = 1.0;
-          if (MODE == 1) {
-#pragma unroll
-            for (int u = 0; u < SP; ++u)
-              if (spj[u] == j) w = spw[u];
-          }
-          double elj, ec;
-          const double sc = pair_term<MODE, EN>(s, ctab, d2, qi * pj[bb].w, epsij, sig6, w, elj, ec);
-          const double px = sc * rx, py = sc * ry, pz = sc * rz;
-          fx += px;
-          fy += py;
-          fz += pz;
-          if (EN) {
-            vir_add(vir, rx, ry, rz, px, py, pz, MODE == 1 ? -0.5 : 0.5);
-            pelj += elj;
-            pec += ec;
-          }
+        double elj, ec;
+        // list entries satisfy the acceptance test with this same vector;
+        // with the minimum image applied the range is tested again
+        const double sc = s.use_min_image ? pair_term<MODE, EN, true>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec)
+                                          : pair_term<MODE, EN, false>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec);
+        const double px = sc * rx, py = sc * ry, pz = sc * rz;
+        fx += px;
+        fy += py;
+        fz += pz;
+        if (EN) {
+          vir_add(vir, rx, ry, rz, px, py, pz, MODE == 1 ? -0.5 : 0.5);
+          pelj += elj;
+          pec += ec;
         }
       }
+      ea = na;
+      eb = nb;
+      pa = qa;
+      pb = qb;
+      ta = ua;
+      tb = ub;
+      na = ma;
+      nb = mb;
     }
   }
   if (THERMO) {
     const double vals[kRowPair] = {pelj, pec, vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
     block_write_row<kRowPair>(vals, rows + (size_t)blockIdx.x * kRowPair);
   }
-  fx = group_sum<L>(fx);
-  fy = group_sum<L>(fy);
-  fz = group_sum<L>(fz);
-  if (PERATOM) {
-    for (int k = 0; k < 6; ++k) vir[k] = group_sum<L>(vir[k]);
-    pelj = group_sum<L>(pelj);
-    pec = group_sum<L>(pec);
+  for (int off = 1; off < L; off <<= 1) {
+    fx += __shfl_xor(fx, off, kWave);
+    fy += __shfl_xor(fy, off, kWave);
+    fz += __shfl_xor(fz, off, kWave);
+    if (PERATOM) {
+      for (int k = 0; k < 6; ++k) vir[k] += __shfl_xor(vir[k], off, kWave);
+      pelj += __shfl_xor(pelj, off, kWave);
+      pec += __shfl_xor(pec, off, kWave);
+    }
   }
   if (i < s.n && sub == 0) {
     s.fnx[i] = fx;
@@ -146,24 +198,22 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
   }
 }
 
-template <int MODE, int LG, int SP>
+template <int MODE, int SP, bool LTAB>
 void launch_pair_v(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int grid) {
   if (peratom)
-    hipLaunchKernelGGL((k_pair_verlet<MODE, true, true, LG, SP>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+    hipLaunchKernelGGL((k_pair_verlet<MODE, true, true, SP, LTAB>), dim3(grid), dim3(kBlock), 0, st, s, rows);
   else if (thermo)
-    hipLaunchKernelGGL((k_pair_verlet<MODE, true, false, LG, SP>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+    hipLaunchKernelGGL((k_pair_verlet<MODE, true, false, SP, LTAB>), dim3(grid), dim3(kBlock), 0, st, s, rows);
   else
-    hipLaunchKernelGGL((k_pair_verlet<MODE, false, false, LG, SP>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+    hipLaunchKernelGGL((k_pair_verlet<MODE, false, false, SP, LTAB>), dim3(grid), dim3(kBlock), 0, st, s, rows);
 }
 
 template <int MODE, int SP>
-void launch_pair_lg(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int grid) {
-  if (s.lanes_lg == 2)
-    launch_pair_v<MODE, 2, SP>(s, st, thermo, peratom, rows, grid);
-  else if (s.lanes_lg == 3)
-    launch_pair_v<MODE, 3, SP>(s, st, thermo, peratom, rows, grid);
+void launch_pair_t(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int grid) {
+  if (s.ntypes <= kMaxTypes)
+    launch_pair_v<MODE, SP, true>(s, st, thermo, peratom, rows, grid);
   else
-    launch_pair_v<MODE, 4, SP>(s, st, thermo, peratom, rows, grid);
+    launch_pair_v<MODE, SP, false>(s, st, thermo, peratom, rows, grid);
 }
 
 // RBL: per-atom corrected force before the mean removal; the virial keeps the
@@ -203,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_rbl(SysView s) {
     for (int k = 0; k < num; ++k) {
       int j;
       double ox, oy, oz;
-      decode(s, s.nbr[nbr_index(base + k, i, s.n, s.lanes_lg)], j, ox, oy, oz);
+      decode(s, s.nbr[nbr_index(base + k, i, s.cap)], j, ox, oy, oz);
       const double4 pj = s.spos[j];
       double rx = (pj.x - ox) - pi.x;
       double ry = (pj.y - oy) - pi.y;
@@ -317,13 +367,13 @@ void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, doub
     return;
   }
   if (s.lj_only)
-    launch_pair_lg<2, 0>(s, st, thermo, peratom, rows, grid);
+    launch_pair_t<2, 0>(s, st, thermo, peratom, rows, grid);
   else if (!s.weighted)
-    launch_pair_lg<0, 0>(s, st, thermo, peratom, rows, grid);
+    launch_pair_t<0, 0>(s, st, thermo, peratom, rows, grid);
   else if (s.sp_max <= 4)
-    launch_pair_lg<1, 4>(s, st, thermo, peratom, rows, grid);
+    launch_pair_t<1, 4>(s, st, thermo, peratom, rows, grid);
   else
-    launch_pair_lg<1, 16>(s, st, thermo, peratom, rows, grid);
+    launch_pair_t<1, 16>(s, st, thermo, peratom, rows, grid);
 }
 
 }  // namespace rbmd

@@ -52,17 +52,35 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
+// P (ISQ false) or Q (ISQ true) of the Coulomb table on interval k at the
+// local coordinate u: Horner over the coefficient pairs of the LDS copy.
+template <bool ISQ>
+__device__ __forceinline__ double coul_poly(const double2* ctab, int k, double u) {
+  const double2* c = ctab + (ISQ ? kCoulPairs * kCoulM : 0) + k;
+  const double2 top = c[(kCoulPairs - 1) * kCoulM];
+  double p = fma(top.y, u, top.x);
+#pragma unroll
+  for (int dp = kCoulPairs - 2; dp >= 0; --dp) {
+    const double2 cc = c[dp * kCoulM];
+    p = fma(p, u, cc.y);
+    p = fma(p, u, cc.x);
+  }
+  return p;
+}
+
 // One pair term; returns the scalar f such that the force on i is f * r.
 // Reference arithmetic: LJ ExecForceFunction.h:412-435 / 818-841, G(r) :682-689,
 // special weights RunWorklet.cpp:2827-2857.  With rinv = 1/r:
 //   LJ     f = -24 eps (2 s6 - 1) s6 rinv^2,    s6 = sigma^6 rinv^6
 //   coul   f = -qq G(r)/r = -qq P(r) rinv^3,     e = qq Q(r) rinv / 2
 //   weight f += (1 - w) qq rinv^3,               e -= (1 - w) qq rinv / 2
-// P and Q come from the LDS copy of the piecewise polynomial table.
-template <int MODE, bool EN>
-__device__ __forceinline__ double pair_term(const SysView& s, const double* ctab, double d2, double qq, double epsij,
+// P and Q come from the LDS copy of the piecewise polynomial table.  CHECK:
+// test 1e-4 < d2 < rc^2 (off for pairs already accepted by the list predicate
+// with the same vector).
+template <int MODE, bool EN, bool CHECK = true>
+__device__ __forceinline__ double pair_term(const SysView& s, const double2* ctab, double d2, double qq, double epsij,
                                             double sig6, double w, double& elj, double& ec) {
-  const bool in = d2 < s.cut2 && d2 > 0.0001;
+  const bool in = !CHECK || (d2 < s.cut2 && d2 > 0.0001);
   double flj = 0, fc = 0;
   elj = 0;
   ec = 0;
@@ -77,10 +95,7 @@ __device__ __forceinline__ double pair_term(const SysView& s, const double* ctab
       int k = (int)fk;
       k = k < kCoulM - 1 ? k : kCoulM - 1;
       const double u = fk - (double)k - 0.5;
-      const double* c = ctab + k * 2 * (kCoulD + 1);
-      double P = c[kCoulD];
-#pragma unroll
-      for (int d = kCoulD - 1; d >= 0; --d) P = fma(P, u, c[d]);
+      const double P = coul_poly<false>(ctab, k, u);
       const double rinv3 = rinv2 * rinv;
       if (MODE == 0) {
         fc = s.qqr2e * (-qq * P * rinv3);
@@ -88,9 +103,7 @@ __device__ __forceinline__ double pair_term(const SysView& s, const double* ctab
         fc = s.qqr2e * (qq * rinv3 * ((1.0 - w) - P));
       }
       if (EN) {
-        double Q = c[2 * kCoulD + 1];
-#pragma unroll
-        for (int d = kCoulD - 1; d >= 0; --d) Q = fma(Q, u, c[kCoulD + 1 + d]);
+        const double Q = coul_poly<true>(ctab, k, u);
         ec = MODE == 0 ? s.qqr2e * (0.5 * qq * Q * rinv) : s.qqr2e * (0.5 * qq * rinv * (Q - (1.0 - w)));
       }
     }

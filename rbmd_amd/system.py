@@ -310,7 +310,7 @@ def write_data(path: str, s: System, atom_style: str = "full", title: str = "rbm
         lines.append(f"1 angle types")
     lines.append("")
     for d, nm in enumerate("xyz"):
-        lines.append(f"{s.lo[d]!r} {s.hi[d]!r} {nm}lo {nm}hi")
+        lines.append(f"{float(s.lo[d])!r} {float(s.hi[d])!r} {nm}lo {nm}hi")
     lines += ["", "Masses", ""]
     tmass = {}
     for i in range(n):
