@@ -314,6 +314,8 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_nbr_build_verlet(SysView 
   const int cc = c / (s.dims[0] * s.dims[1]);
   const int a = wave / 3 - 1, b = wave % 3 - 1;
   const int seg0 = wave * s.seg_cap;
+  const int seg_cap = s.seg_cap;
+  int* row = s.nbr + nbr_index(seg0, active ? i : 0, s.cap);  // this lane's segment
   Column col;
   make_column(s, ca, cb, cc, a, b, lane, col);
   const int total = col.total;
@@ -344,30 +346,27 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_nbr_build_verlet(SysView 
       stage_e[wave][slot] = entry;
     }
     wave_sync();
-    unsigned long long acc = 0;
+    // test every staged candidate; an accepting lane appends the candidate's
+    // entry (an LDS broadcast) to its own row at once, so entries stay in
+    // stream order and no per-lane store loop follows
 #pragma unroll 4
     for (int u = 0; u < kc; ++u) {
       const double4 q = stage[wave][u];
+      const int e = stage_e[wave][u];
       const double rx = q.x - pi.x;
       const double ry = q.y - pi.y;
       const double rz = q.z - pi.z;
       const double d2 = rx * rx + ry * ry + rz * rz;
       const bool ok = (cut2 - d2 > 0.0001) & (d2 > 0.0001);
-      acc |= (unsigned long long)ok << u;
-    }
-    if (!active) acc = 0;
-    while (acc) {
-      const int u = __builtin_ctzll(acc);
-      acc &= acc - 1;
-      if (cnt < s.seg_cap)
-        s.nbr[nbr_index(seg0 + cnt, i, s.cap)] = stage_e[wave][u];
-      else
-        atomicOr(s.status, 4);
-      cnt++;
+      if (active & ok) {
+        if (cnt < seg_cap) row[cnt] = e;
+        ++cnt;
+      }
     }
     wave_sync();
   }
-  if (active) s.nbr_num[wave * s.n + i] = cnt < s.seg_cap ? cnt : s.seg_cap;
+  if (cnt > seg_cap) atomicOr(s.status, 4);
+  if (active) s.nbr_num[wave * s.n + i] = cnt < seg_cap ? cnt : seg_cap;
 }
 
 // RBL list state: core slots [0, rs_num), kept shell slots from rs_num (every
