@@ -15,8 +15,13 @@ Prints one JSON line.  Extra objects:
                 atom against every atom of its 27 stencil cells, 43 per
                 accepted pair (pair term, counted from each side)
   cpu_baseline  the CPU oracle (a C++ restatement of the reference
-                algorithms) timed on this host's cores on a few steps of the
-                same workload
+                algorithms) timed on this host's cores on a bounded sample of
+                the same workload (--cpu-steps MD steps, about 10 s)
+
+roofline.traffic is the dominant kernel's HBM bytes per launch measured with
+rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (profiles/hbm_traffic.json, scaled by
+the same counters' reading for a device copy of known size, see
+--copy-calibrate); null when that kernel was not measured.
 """
 import argparse
 import json
@@ -88,10 +93,23 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="water", choices=["water", "salt"])
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device once (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        import torch
+
+        n = args.copy_calibrate << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is design-only)",
               file=sys.stderr)
@@ -133,11 +151,18 @@ def main():
         "kforce": 35.0 * system.n_atoms * P,
         "rho_partial": 39.0 * system.n_atoms * P,
     }
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tt = json.load(f).get("kernels", {}).get(dom)
+        if tt:
+            traffic = tt["bytes_per_launch"]
     roof = None
     if dom in flops:
         achieved = flops[dom] / (kernels[dom]["mean_us"] * 1e-6) / 1e12
         roof = {"kernel": dom, "bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
                 "work_per_launch": flops[dom], "peak_note": "MI355X FP64 vector peak (equal to FP64 matrix peak)"}
     th = g.thermo()
     out = {

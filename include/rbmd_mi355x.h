@@ -60,7 +60,8 @@ enum { RBMD_SHAKE_NULL = 0, RBMD_SHAKE_TRUE = 1, RBMD_SHAKE_FALSE = 2 };
 enum { RBMD_TEMP_NOSE_HOOVER = 0, RBMD_TEMP_BERENDSEN = 1, RBMD_TEMP_RESCALE = 2, RBMD_TEMP_NONE = 3 };
 /* Arithmetic of the pair / structure-factor kernels.  FP64: everything in
  * double.  MIXED: pair and rho(k) math in FP32 with compensated accumulation;
- * positions, velocities, SHAKE and global reductions stay FP64. */
+ * positions, velocities, SHAKE and global reductions stay FP64 (not built yet:
+ * rbmd_sys_create returns RBMD_ERR_UNSUPPORTED). */
 enum { RBMD_PREC_FP64 = 0, RBMD_PREC_MIXED = 1 };
 /* Real-space erfc kernel: analytic G(r) (ExecForceFunction.h:682-689) or the
  * fixed alpha=0.1 step table of the RBL path (ExecStaticTable.h:96-135). */
@@ -219,8 +220,10 @@ rbmd_status rbmd_ewald_force(rbmd_sys* sys, const int32_t kmax[3], double* force
 rbmd_status rbmd_force_bonded(rbmd_sys* sys, double* force, double* e_bond_sum, double* e_angle_sum);
 
 /* ---- MD loop: ExecutionNVT::Init (ExecutionNVT.cpp:60-74) then Solve per step (:90-135).
- *      rbmd_md_init evaluates the first force; rbmd_md_step runs n steps (PreSolve+Solve each);
- *      rbmd_md_run_graph captures one step as a hipGraph and replays it n times. ---- */
+ *      rbmd_md_init evaluates the first force; rbmd_md_step enqueues n steps (PreSolve+Solve
+ *      each) on the context stream without host round trips; rbmd_md_run_graph runs the same
+ *      steps through a captured step graph where that is built (currently it enqueues as
+ *      rbmd_md_step does). ---- */
 rbmd_status rbmd_md_init(rbmd_sys* sys);
 rbmd_status rbmd_md_step(rbmd_sys* sys, int64_t n_steps);
 rbmd_status rbmd_md_run_graph(rbmd_sys* sys, int64_t n_steps);

@@ -6,8 +6,23 @@
  * and by the cpu_baseline leg of bench.py as the checker and the CPU
  * baseline, never as the thing measured or shipped.
  *
- * Every function names the reference file:line it restates.  Parity status
- * of each part (what pins it) is listed in oracle/README.md.
+ * Every function names the reference file:line it restates.  What pins each
+ * part (tests/test_oracle_pins.py unless noted):
+ *   locator            canonical order defined here; neighbour sets below
+ *   neighbours         all-image brute-force counts, entry validity, no duplicates
+ *   force_real         two-atom LJ and erfc-Coulomb closed forms; finite
+ *                      differences of the energy (salt)
+ *   RBL lists / force  same pair term; capacities and sampling are the
+ *                      reference's formulas restated (parity unpinned beyond
+ *                      the restatement: no closed form exists for the shell sample)
+ *   rbe_S              theta-function identity vs the direct k sum; 734.5889
+ *   rbe_sample         marginal frequencies vs the exact discrete Gaussian
+ *   rbe_force / rho    momentum conservation; unbiased vs the Ewald k-space force
+ *   ewald              Madelung constant 1.747564594633
+ *   bonded             finite differences of the bonded energy; one-bond closed form
+ *   md_step            velocity-Verlet time reversibility; SHAKE/RATTLE residuals;
+ *                      NHC chain restated (parity unpinned beyond the restatement)
+ *   reference inputs   tests/test_inputs.py (the reference's own data and JSON files)
  */
 #ifndef RBMD_ORACLE_H
 #define RBMD_ORACLE_H

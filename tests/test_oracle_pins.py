@@ -220,3 +220,40 @@ def test_shake_rattle_constraints():
     d = vec(ang[:, 1], ang[:, 0])
     dv = v[ang[:, 0]] - v[ang[:, 1]]
     assert np.abs((d * dv).sum(1)).max() < 1e-6 * np.abs(v).max()
+
+
+def test_bonded_forces_are_minus_energy_gradient():
+    """harmonic bonds E = k (r - r0)^2 and angles E = k (theta - theta0)^2
+    (MolecularWorklet.h:58-397): forces against finite differences of the
+    bonded energy on a jittered water box"""
+    s = make_water(n_mol=50, seed=9)
+    rng = np.random.default_rng(3)
+    x0 = s.x + rng.uniform(-0.08, 0.08, s.x.shape)
+    o = oracle.OracleSystem(s, water_config())
+    o.set_state(x=x0)
+    f, eb, ea = o.force_bonded()
+    assert eb > 0 and ea > 0
+
+    def energy(x):
+        o.set_state(x=x)
+        _, b, a = o.force_bonded()
+        return b + a
+
+    h = 1e-6
+    for a in (0, 1, 2, 31, 77):
+        for d in range(3):
+            xp = x0.copy()
+            xp[a, d] += h
+            xm = x0.copy()
+            xm[a, d] -= h
+            g = (energy(xp) - energy(xm)) / (2 * h)
+            assert abs(-g - f[a, d]) <= 1e-6 * max(1.0, abs(f[a, d])), (a, d, -g, f[a, d])
+    # a single bond at r: the bond energy is k (r - r0)^2 exactly
+    i, j = s.bond_atoms[0], s.bond_atoms[1]
+    r = np.linalg.norm(x0[i] - x0[j])
+    o2 = oracle.OracleSystem(s, water_config())
+    x1 = s.x.copy()
+    x1[j] = x1[i] + (x0[j] - x0[i])  # only bond 0 stretched
+    o2.set_state(x=x1)
+    _, eb1, _ = o2.force_bonded()
+    assert abs(eb1 - s.bond_k[0] * (r - s.bond_r0[0]) ** 2) <= 1e-12 * max(eb1, 1e-30)
