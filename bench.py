@@ -87,15 +87,68 @@ def cpu_baseline(system, run, steps, threads):
     return system.n_atoms * steps / dt, dt
 
 
+def bench_ensemble(args):
+    """Packed ensemble (graph steps on several streams) against the plain
+    one-system-at-a-time FP64 loop on the same systems."""
+    from rbmd_amd.serve import ensemble_specs, run_packed, run_plain
+
+    specs = ensemble_specs(args.systems, args.ens_steps, seed=2024, n_max=args.max_atoms)
+    print(f"[bench] ensemble: {len(specs)} systems, {sum(s.atoms for s in specs)} atoms, "
+          f"{args.ens_steps} steps each", file=sys.stderr, flush=True)
+    plain = run_plain(specs)
+    print(f"[bench] plain loop {plain['atom_steps_per_s']:.4g} atom-steps/s", file=sys.stderr, flush=True)
+    packed = run_packed(specs, streams=args.streams, chunk=args.chunk)
+    out = {"metric": "atom-steps/s (ensemble, aggregate)", "value": round(packed["atom_steps_per_s"], 1),
+           "unit": "atom-steps/s", "n_gpus": 1, "steps": args.ens_steps, "warmup": 0,
+           "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{len(specs)} independent systems, half charged-LJ salt, half rigid water, "
+                                  f"atoms log-uniform in [600, {args.max_atoms}], {args.ens_steps} steps each",
+                      "streams": args.streams, "chunk": args.chunk},
+           "plain_loop_atom_steps_per_s": round(plain["atom_steps_per_s"], 1),
+           "speedup_vs_plain_loop": round(packed["atom_steps_per_s"] / plain["atom_steps_per_s"], 3)}
+    print(json.dumps(out))
+
+
+def bench_serve(args):
+    """Scaled served stream: Poisson arrivals, admission by footprint,
+    continuous batching; p50 / p99 latency and aggregate throughput."""
+    from rbmd_amd.serve import build, serve, stream_specs
+
+    jobs = stream_specs(args.rate, args.duration, seed=77, n_max=args.max_atoms, steps_min=200,
+                        steps_max=args.max_steps)
+    print(f"[bench] serve: {len(jobs)} jobs over {args.duration} s", file=sys.stderr, flush=True)
+    inputs = {j: build(sp) for j, sp in enumerate(jobs)}  # each job's input as it arrives
+    r = serve(jobs, streams=args.streams, chunk=args.chunk, prebuilt=inputs)
+    out = {"metric": "served jobs: p50/p99 latency, aggregate atom-steps/s", "value": round(r["atom_steps_per_s"], 1),
+           "unit": "atom-steps/s", "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"Poisson arrivals {args.rate}/s for {args.duration} s, salt or water 50/50, "
+                                  f"atoms log-uniform in [1000, {args.max_atoms}], steps uniform in "
+                                  f"[200, {args.max_steps}]", "streams": args.streams, "chunk": args.chunk},
+           "jobs": r["jobs"], "p50_latency_s": r["p50_latency_s"], "p99_latency_s": r["p99_latency_s"],
+           "wall_s": round(r["wall_s"], 3)}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="water", choices=["water", "salt"])
+    ap.add_argument("--workload", default="water", choices=["water", "salt", "ensemble", "serve"])
+    ap.add_argument("--systems", type=int, default=64, help="ensemble: number of systems (half salt, half water)")
+    ap.add_argument("--ens-steps", type=int, default=200, help="ensemble: MD steps per system")
+    ap.add_argument("--streams", type=int, default=8, help="ensemble / serve: contexts (HIP streams)")
+    ap.add_argument("--chunk", type=int, default=50, help="ensemble / serve: graph steps per enqueue round")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrivals per second")
+    ap.add_argument("--duration", type=float, default=5.0, help="serve: seconds of arrivals")
+    ap.add_argument("--max-atoms", type=int, default=30000, help="ensemble / serve: largest system")
+    ap.add_argument("--max-steps", type=int, default=500, help="serve: most steps of a job")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--graph", action="store_true", help="replay a captured step graph (rbmd_md_run_graph)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -117,15 +170,20 @@ def main():
 
     from rbmd_amd.lib import Context, GpuSystem
 
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     system, run, desc, dt_fs = workload(args.workload)
     print(f"[bench] workload {args.workload}: {system.n_atoms} atoms", file=sys.stderr, flush=True)
     ctx = Context(0)
     g = GpuSystem(ctx, system, run)
     g.md_init()
-    g.md_step(args.warmup)
+    run_steps = g.md_run_graph if args.graph else g.md_step
+    run_steps(args.warmup)
     ctx.synchronize()
     t0 = time.perf_counter()
-    g.md_step(args.steps)
+    run_steps(args.steps)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
@@ -153,7 +211,7 @@ def main():
     }
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.workload == "water":  # measured on the water workload only
         with open(tpath) as f:
             tt = json.load(f).get("kernels", {}).get(dom)
         if tt:

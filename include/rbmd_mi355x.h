@@ -227,6 +227,14 @@ rbmd_status rbmd_force_bonded(rbmd_sys* sys, double* force, double* e_bond_sum, 
 rbmd_status rbmd_md_init(rbmd_sys* sys);
 rbmd_status rbmd_md_step(rbmd_sys* sys, int64_t n_steps);
 rbmd_status rbmd_md_run_graph(rbmd_sys* sys, int64_t n_steps);
+/* Enqueue-only variant for many systems in flight (no counterpart in the reference,
+ * whose executioner runs one system): n graph-replayed steps are enqueued on the
+ * context stream and the call returns at once; rbmd_md_wait synchronises and
+ * returns the status of everything enqueued since; rbmd_md_query reports 1 when
+ * the context stream is idle, 0 while work is pending. */
+rbmd_status rbmd_md_enqueue(rbmd_sys* sys, int64_t n_steps);
+rbmd_status rbmd_md_wait(rbmd_sys* sys);
+rbmd_status rbmd_md_query(rbmd_sys* sys, int32_t* idle);
 rbmd_status rbmd_md_thermo(rbmd_sys* sys, rbmd_thermo* out);
 rbmd_status rbmd_md_energy(rbmd_sys* sys); /* evaluate e_long now (ComputeEwaldEnergy) */
 

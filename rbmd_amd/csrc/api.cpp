@@ -169,6 +169,9 @@ struct rbmd_sys {
   int P = 0;
   double S = 0;
   double* kcur = nullptr;  // [3P] current k vectors
+  double* kgraph = nullptr;  // [3P] k vectors of the running step inside a step graph
+  hipGraphExec_t graph = nullptr;  // one captured MD step (rbmd_md_run_graph)
+  bool graph_ok = false;           // cleared whenever captured pointers or sizes change
   double* kring = nullptr; // [2R][3P]
   double* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -366,6 +369,7 @@ rbmd_status ensure_nbr(rbmd_sys* s, int cap) {
   s->v.nseg = nseg;
   s->v.seg_cap = seg_cap;
   s->v.cap = nseg * seg_cap;
+  s->graph_ok = false;
   return s->dalloc(&s->v.nbr, (size_t)s->v.cap * s->n);
 }
 
@@ -637,7 +641,15 @@ rbmd_status rbmd_footprint_estimate(const rbmd_sys_desc* d, const rbmd_params* p
   double c = d->n_atoms / vol * 4.0 / 3.0 * kPi * p->cut_off * p->cut_off * p->cut_off;
   int64_t n = d->n_atoms;
   int64_t cells = (int64_t)dims[0] * dims[1] * dims[2];
-  int64_t b = n * (8 * 36 + 4 * 12) + cells * 16 + (int64_t)(2.0 * c + 64) * n * 4;
+  // list slots per atom as ensure_nbr sizes them (VERLET: 9 column segments)
+  int64_t slots;
+  if (p->neighbor_type == RBMD_NEIGHBOR_RBL) {
+    slots = (int64_t)(2.0 * c + 64);
+  } else {
+    const double f = column_fraction(dx[0], dx[1], p->cut_off);
+    slots = 9 * (int64_t)round_up((int)(1.15 * f * c + 64), kMaxLanes);
+  }
+  int64_t b = n * (8 * 60 + 4 * 24) + cells * 16 + slots * n * 4;
   b += (int64_t)3 * std::max(p->rbe_p, 1) * 8 * 512 + ((n + 1023) / 1024) * 2 * std::max(p->rbe_p, 1) * 8;
   *bytes = b;
   return RBMD_OK;
@@ -988,6 +1000,7 @@ rbmd_status rbmd_sys_destroy(rbmd_sys* s) {
   if (!s) return RBMD_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
   for (void* q : s->allocs) (void)hipFree(q);
   for (int h = 0; h < 2; ++h) {
     if (s->stage[h]) (void)hipHostFree(s->stage[h]);
@@ -1242,7 +1255,9 @@ rbmd_status rbmd_rbe_set_kvectors(rbmd_sys* s, int32_t p, const double* m) {
     s->P = p;
     RC_TRY(s->dalloc(&s->kcur, 3 * (size_t)p));
     RC_TRY(ensure_rho(s, p));
+    s->kgraph = nullptr;
   }
+  s->graph_ok = false;
   std::vector<double> k(3 * (size_t)p);
   for (int l = 0; l < p; ++l)
     for (int d = 0; d < 3; ++d) k[3 * l + d] = 2 * kPi * m[3 * l + d] / s->v.box[d];
@@ -1380,6 +1395,7 @@ rbmd_status rbmd_md_init(rbmd_sys* s) {
   hipStream_t st = c->stream;
   launch_fill_int(s->v.status, 0, 1, st);
   s->injected = false;
+  s->graph_ok = false;
   NhcState zero;
   std::memset(&zero, 0, sizeof(zero));
   HIP_TRY(hipMemcpyAsync(s->nhc, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
@@ -1404,16 +1420,23 @@ rbmd_status rbmd_md_init(rbmd_sys* s) {
   return status_to_rc(stt);
 }
 
-static rbmd_status md_one_step(rbmd_sys* s) {
+// k vectors come from the host-filled ring (not from injected or Ewald sets)
+static bool kring_on(const rbmd_sys* s) {
+  return has_k(s) && s->p.coulomb_type == RBMD_COULOMB_RBE && !s->injected;
+}
+
+// Enqueue one MD step (ExecutionNVT::PreSolve + Solve, ExecutionNVT.cpp:76-135)
+// with the k vectors at kv.  from_ring: inside a step graph; the step's k
+// vectors are copied from the ring slot the device step counter names.
+static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring) {
   rbmd_ctx* c = s->ctx;
   hipStream_t st = c->stream;
-  const int64_t step = s->step + 1;
-  const double* kv = nullptr;
-  RC_TRY(kvec_for_step(s, step, &kv));
   const int tc = s->p.temp_ctrl;
   const bool nh = tc == RBMD_TEMP_NOSE_HOOVER || tc == RBMD_TEMP_NONE;
   const bool shake = shake_on(s);
   c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, false), nh ? 1 : 5, st); });
+  if (from_ring)  // the nhc phase above advanced the device step counter
+    c->run("kvec_slot", [&] { launch_kvec_slot(s->kgraph, s->kring, s->ring_R, 3 * s->P, &s->nhc->step, st); });
   const double* scale = (tc == RBMD_TEMP_NOSE_HOOVER) ? &s->nhc->factor : nullptr;
   c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, 1); });
   build_cells(s, false);
@@ -1445,7 +1468,40 @@ static rbmd_status md_one_step(rbmd_sys* s) {
     c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 4, st); });
     c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
   }
+}
+
+static rbmd_status md_one_step(rbmd_sys* s) {
+  const int64_t step = s->step + 1;
+  const double* kv = nullptr;
+  RC_TRY(kvec_for_step(s, step, &kv));
+  enqueue_step(s, kv, false);
   s->step = step;
+  return RBMD_OK;
+}
+
+// Capture one step into a graph.  The captured launches carry this system's
+// buffers and sizes, so the graph is rebuilt after anything changes them.
+static rbmd_status capture_step(rbmd_sys* s) {
+  rbmd_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  if (s->graph) {
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+  }
+  const bool ring = kring_on(s);
+  if (ring && !s->kgraph) RC_TRY(s->dalloc(&s->kgraph, 3 * (size_t)s->P));
+  const bool prof = c->prof;
+  c->prof = false;
+  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  enqueue_step(s, ring ? s->kgraph : s->kcur, ring);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  c->prof = prof;
+  if (e != hipSuccess) return fail(RBMD_ERR_HIP, "step capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(RBMD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  s->graph_ok = true;
   return RBMD_OK;
 }
 
@@ -1465,7 +1521,58 @@ rbmd_status rbmd_md_step(rbmd_sys* s, int64_t n_steps) {
   return status_to_rc(stt);
 }
 
-rbmd_status rbmd_md_run_graph(rbmd_sys* s, int64_t n_steps) { return rbmd_md_step(s, n_steps); }
+// Same steps as rbmd_md_step, each replayed from one captured step graph: the
+// host only tops up the k-vector ring (every R steps) and launches the graph.
+static rbmd_status enqueue_graph_steps(rbmd_sys* s, int64_t n_steps) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (!s->initialized) return fail(RBMD_ERR_STATE, "rbmd_md_init has not run");
+  if (!s->graph_ok) RC_TRY(capture_step(s));
+  hipStream_t st = s->ctx->stream;
+  const bool ring = kring_on(s);
+  for (int64_t k = 0; k < n_steps; ++k) {
+    const int64_t step = s->step + 1;
+    if (ring) {
+      const double* unused = nullptr;
+      RC_TRY(kvec_for_step(s, step, &unused));  // fills the ring half this step reads
+    }
+    HIP_TRY(hipGraphLaunch(s->graph, st));
+    s->step = step;
+    if (s->p.energy_every > 0 && has_k(s) && s->step % s->p.energy_every == 0) {
+      RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
+      s->energy_step = s->step;
+    }
+  }
+  return check_launch("md_run_graph");
+}
+
+rbmd_status rbmd_md_run_graph(rbmd_sys* s, int64_t n_steps) {
+  RC_TRY(enqueue_graph_steps(s, n_steps));
+  int stt = 0;
+  RC_TRY(read_status(s, &stt));
+  return status_to_rc(stt);
+}
+
+rbmd_status rbmd_md_enqueue(rbmd_sys* s, int64_t n_steps) { return enqueue_graph_steps(s, n_steps); }
+
+rbmd_status rbmd_md_wait(rbmd_sys* s) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  int stt = 0;
+  RC_TRY(read_status(s, &stt));
+  return status_to_rc(stt);
+}
+
+rbmd_status rbmd_md_query(rbmd_sys* s, int32_t* idle) {
+  if (!s || !idle) return fail(RBMD_ERR_ARG, "null argument");
+  hipError_t e = hipStreamQuery(s->ctx->stream);
+  if (e == hipSuccess) {
+    *idle = 1;
+  } else if (e == hipErrorNotReady) {
+    *idle = 0;
+  } else {
+    return fail(RBMD_ERR_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
+  }
+  return RBMD_OK;
+}
 
 rbmd_status rbmd_md_thermo(rbmd_sys* s, rbmd_thermo* o) {
   if (!s || !o) return fail(RBMD_ERR_ARG, "bad arguments");

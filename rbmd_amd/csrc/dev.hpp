@@ -158,6 +158,7 @@ void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st);
 void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, int thermo,
                    int peratom, double* partials, int* nblocks);
 int kforce_blocks(const SysView& s);
+void launch_kvec_slot(double* dst, const double* ring, int R, int per, const int64_t* step, hipStream_t st);
 
 // ---------------------------------------------------------------- integrate
 struct IntView {

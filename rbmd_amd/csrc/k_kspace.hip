@@ -92,6 +92,20 @@ __global__ __launch_bounds__(kBlock) void k_rho_reduce(KView kv) {
   kv.rho[t] = acc;
 }
 
+// the k vectors of the current step (device step counter) from the
+// double-buffered ring [2][R][3P] into a fixed buffer (step graphs read it)
+__global__ __launch_bounds__(kBlock) void k_kvec_slot(double* dst, const double* ring, int R, int per,
+                                                      const int64_t* step) {
+  const int64_t st = *step;
+  const size_t slot = (size_t)((st / R) % 2) * R + (size_t)(st % R);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < per; t += gridDim.x * blockDim.x)
+    dst[t] = ring[slot * per + t];
+}
+
+void launch_kvec_slot(double* dst, const double* ring, int R, int per, const int64_t* step, hipStream_t st) {
+  hipLaunchKernelGGL(k_kvec_slot, dim3((per + kBlock - 1) / kBlock), dim3(kBlock), 0, st, dst, ring, R, per, step);
+}
+
 void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st) {
   dim3 grid(k.nblk_rho, (k.P + kRhoK - 1) / kRhoK);
   hipLaunchKernelGGL(k_rho_partial, grid, dim3(kBlock), 0, st, s, k);

@@ -24,7 +24,7 @@ EXPORTS = [
     "rbmd_locator_export", "rbmd_neighbors_build", "rbmd_neighbors_count", "rbmd_neighbors_capacity", "rbmd_neighbors_export", "rbmd_force_real",
     "rbmd_rbe_set_kvectors", "rbmd_rbe_sample", "rbmd_rbe_sample_host", "rbmd_rbe_S",
     "rbmd_rbe_structure_factor", "rbmd_rbe_force", "rbmd_ewald_force", "rbmd_force_bonded", "rbmd_md_init",
-    "rbmd_md_step", "rbmd_md_run_graph", "rbmd_md_thermo", "rbmd_md_energy", "rbmd_profile_enable",
+    "rbmd_md_step", "rbmd_md_run_graph", "rbmd_md_enqueue", "rbmd_md_wait", "rbmd_md_query", "rbmd_md_thermo", "rbmd_md_energy", "rbmd_profile_enable",
     "rbmd_profile_reset", "rbmd_profile_get", "rbmd_abi_sizeof",
 ]
 
@@ -75,6 +75,9 @@ def load():
     L.rbmd_md_init.argtypes = [vp]
     L.rbmd_md_step.argtypes = [vp, C.c_int64]
     L.rbmd_md_run_graph.argtypes = [vp, C.c_int64]
+    L.rbmd_md_enqueue.argtypes = [vp, C.c_int64]
+    L.rbmd_md_wait.argtypes = [vp]
+    L.rbmd_md_query.argtypes = [vp, C.POINTER(C.c_int32)]
     L.rbmd_md_thermo.argtypes = [vp, C.POINTER(Thermo)]
     L.rbmd_md_energy.argtypes = [vp]
     L.rbmd_profile_enable.argtypes = [vp, C.c_int32]
@@ -188,6 +191,15 @@ class GpuSystem:
         check(self.L.rbmd_sys_footprint(self.h, C.byref(b)))
         return b.value
 
+    @staticmethod
+    def estimate_footprint(system, run):
+        """device bytes a system would hold (rbmd_footprint_estimate; no device needed)"""
+        d = DescHolder(system)
+        p = make_params(run)
+        b = C.c_int64()
+        check(load().rbmd_footprint_estimate(C.byref(d.desc), C.byref(p), C.byref(b)))
+        return b.value
+
     def set_state(self, x=None, v=None):
         xx = None if x is None else np.ascontiguousarray(x, np.float64).reshape(-1)
         vv = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
@@ -288,6 +300,21 @@ class GpuSystem:
 
     def md_step(self, n):
         return check(self.L.rbmd_md_step(self.h, int(n)))
+
+    def md_run_graph(self, n):
+        return check(self.L.rbmd_md_run_graph(self.h, int(n)))
+
+    def md_enqueue(self, n):
+        """enqueue n graph-replayed steps and return at once (see md_wait)"""
+        return check(self.L.rbmd_md_enqueue(self.h, int(n)))
+
+    def md_wait(self):
+        return check(self.L.rbmd_md_wait(self.h))
+
+    def md_idle(self):
+        idle = C.c_int32()
+        check(self.L.rbmd_md_query(self.h, C.byref(idle)))
+        return bool(idle.value)
 
     def thermo(self):
         t = Thermo()

@@ -183,24 +183,33 @@ def water_geometry(rng: np.random.Generator, n: int, bond: float = 1.0, angle_de
 
 
 def rigidify_velocities(x: np.ndarray, v: np.ndarray, mass: np.ndarray, L: float, n_mol: int) -> np.ndarray:
-    """Project every molecule's velocities onto rigid-body motion (COM + rotation)."""
-    out = v.copy()
-    for mol in range(n_mol):
-        ids = [3 * mol, 3 * mol + 1, 3 * mol + 2]
-        r = x[ids].copy()
-        r[1:] -= L * np.round((r[1:] - r[0]) / L)
-        m = mass[ids]
-        M = m.sum()
-        rc = (m[:, None] * r).sum(0) / M
-        vc = (m[:, None] * v[ids]).sum(0) / M
-        d = r - rc
-        Lang = (m[:, None] * np.cross(d, v[ids] - vc)).sum(0)
-        I = np.zeros((3, 3))
-        for k in range(3):
-            I += m[k] * (np.dot(d[k], d[k]) * np.eye(3) - np.outer(d[k], d[k]))
-        om = np.linalg.solve(I, Lang)
-        out[ids] = vc + np.cross(om, d)
-    return out
+    """Project every molecule's velocities onto rigid-body motion (COM + rotation),
+    all molecules at once (atoms 3k, 3k+1, 3k+2 form molecule k)."""
+    r = x.reshape(n_mol, 3, 3).copy()
+    r[:, 1:] -= L * np.round((r[:, 1:] - r[:, :1]) / L)
+    vv = v.reshape(n_mol, 3, 3)
+    m = mass.reshape(n_mol, 3)
+    M = m.sum(1)
+    rc = (m[:, :, None] * r).sum(1) / M[:, None]
+    vc = (m[:, :, None] * vv).sum(1) / M[:, None]
+    d = r - rc[:, None, :]
+    Lang = (m[:, :, None] * np.cross(d, vv - vc[:, None, :])).sum(1)
+    dd = (d * d).sum(2)
+    I = (m * dd).sum(1)[:, None, None] * np.eye(3) - np.einsum("mk,mki,mkj->mij", m, d, d)
+    om = np.linalg.solve(I, Lang[:, :, None])[:, :, 0]
+    return (vc[:, None, :] + np.cross(om[:, None, :], d)).reshape(-1, 3)
+
+
+def water_special_lists(n_mol: int, special_bonds) -> tuple:
+    """special_lists() for molecules bonded O-H1, O-H2 (bonds listed in that
+    order), built directly: O lists H1, H2 (1-2); each H lists O (1-2) and the
+    other H (1-3).  Same order and weights as the generic walk."""
+    w0, w1 = float(int(special_bonds[0])), float(int(special_bonds[1]))
+    base = 3 * np.arange(n_mol)
+    ids = np.stack([base + 1, base + 2, base, base + 2, base, base + 1], 1).reshape(-1).astype(np.int32)
+    w = np.tile(np.array([w0, w0, w0, w1, w0, w1]), n_mol)
+    off = (2 * np.arange(3 * n_mol + 1)).astype(np.int32)
+    return off, ids, w
 
 
 def make_water(n_mol: int = 10000, density_g_cm3: float = 1.0, seed: int = 1, temperature: float = 300.0,
@@ -230,30 +239,43 @@ def make_water(n_mol: int = 10000, density_g_cm3: float = 1.0, seed: int = 1, te
     grid = -np.ones((m, m, m), dtype=np.int64)
     sc = sites[chosen]
     grid[sc[:, 0], sc[:, 1], sc[:, 2]] = np.arange(n_mol)
-    for _ in range(200):
+    offsets = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1) if (dx, dy, dz) != (0, 0, 0)]
+
+    def too_close(idx):
+        """molecules of idx with an intermolecular contact below min_dist"""
         pos = np.stack([O, O + h1, O + h2], 1)  # [n,3,3]
-        bad = np.zeros(n_mol, bool)
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    if dx == dy == dz == 0:
-                        continue
-                    nb = grid[(sc[:, 0] + dx) % m, (sc[:, 1] + dy) % m, (sc[:, 2] + dz) % m]
-                    ok = nb >= 0
-                    if not ok.any():
-                        continue
-                    ii = np.flatnonzero(ok)
-                    pj = pos[nb[ii]]
-                    pi = pos[ii]
-                    d = pi[:, :, None, :] - pj[:, None, :, :]
-                    d -= L * np.round(d / L)
-                    dmin = np.sqrt((d * d).sum(-1)).reshape(len(ii), -1).min(1)
-                    bad[ii[dmin < min_dist]] = True
-        if not bad.any():
+        bad = np.zeros(len(idx), bool)
+        for dx, dy, dz in offsets:
+            nb = grid[(sc[idx, 0] + dx) % m, (sc[idx, 1] + dy) % m, (sc[idx, 2] + dz) % m]
+            ok = nb >= 0
+            if not ok.any():
+                continue
+            ii = np.flatnonzero(ok)
+            pj = pos[nb[ii]]
+            pi = pos[idx[ii]]
+            d = pi[:, :, None, :] - pj[:, None, :, :]
+            d -= L * np.round(d / L)
+            dmin = np.sqrt((d * d).sum(-1)).reshape(len(ii), -1).min(1)
+            bad[ii[dmin < min_dist]] = True
+        return idx[bad]
+
+    def with_neighbours(idx):
+        out = [idx]
+        for dx, dy, dz in offsets:
+            nb = grid[(sc[idx, 0] + dx) % m, (sc[idx, 1] + dy) % m, (sc[idx, 2] + dz) % m]
+            out.append(nb[nb >= 0])
+        return np.unique(np.concatenate(out))
+
+    # the contact test is symmetric, so after re-orienting the flagged
+    # molecules only they and their lattice neighbours can change status
+    bad = too_close(np.arange(n_mol))
+    for _ in range(199):
+        if len(bad) == 0:
             break
-        nh1, nh2 = water_geometry(rng, int(bad.sum()))
+        nh1, nh2 = water_geometry(rng, len(bad))
         h1[bad] = nh1
         h2[bad] = nh2
+        bad = too_close(with_neighbours(bad))
     x = np.stack([O, O + h1, O + h2], 1).reshape(-1, 3)
     x = np.mod(x, L)
     n = 3 * n_mol
@@ -272,7 +294,7 @@ def make_water(n_mol: int = 10000, density_g_cm3: float = 1.0, seed: int = 1, te
     v -= (mass[:, None] * v).sum(0) / mass.sum()
     t_now = float((mass[:, None] * v * v).sum()) * mvv2e / (dof * kB)
     v *= math.sqrt(temperature / t_now)
-    so, si, sw = special_lists(n, bonds, [0, 0, 0])
+    so, si, sw = water_special_lists(n_mol, [0, 0, 0])
     return System(lo=np.zeros(3), hi=np.full(3, L), x=x, v=v, type=typ, charge=q, mass=mass,
                   eps=np.array([0.1553, 0.0]), sigma=np.array([3.166, 0.0]), mol_id=mol,
                   special_offsets=so, special_ids=si, special_weights=sw,

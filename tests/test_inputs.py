@@ -80,3 +80,28 @@ def test_write_read_round_trip(tmp_path):
     assert np.array_equal(r.type, s.type) and np.array_equal(r.charge, s.charge)
     assert np.array_equal(r.bond_atoms, s.bond_atoms) and np.array_equal(r.angle_atoms, s.angle_atoms)
     assert np.array_equal(r.special_ids, s.special_ids)
+
+
+def test_water_special_lists_fast_path_matches_generic():
+    from rbmd_amd.system import water_special_lists
+
+    n_mol = 7
+    base = 3 * np.arange(n_mol)
+    bonds = np.stack([base, base + 1, base, base + 2], 1).reshape(-1).astype(np.int32)
+    for sb in ([0, 0, 0], [0, 1, 1], [2, 3, 4]):
+        a = special_lists(3 * n_mol, bonds, sb)
+        b = water_special_lists(n_mol, sb)
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
+
+
+def test_rigid_water_velocities():
+    s = make_water(n_mol=200, seed=2)
+    L = s.box[0]
+    x = s.x.reshape(-1, 3, 3)
+    v = s.v.reshape(-1, 3, 3)
+    d = x[:, 1:] - x[:, :1]
+    d -= L * np.round(d / L)
+    dv = v[:, 1:] - v[:, :1]
+    # rigid motion: no stretching along the O-H bonds
+    assert np.abs((d * dv).sum(2)).max() < 1e-12 * max(1.0, np.abs(v).max())
