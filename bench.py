@@ -33,29 +33,48 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 PAIR_FLOPS = 43.0  # per accepted pair (counted from each side): LJ + erfc-Coulomb force
 TEST_FLOPS = 8.0  # per distance test of an atom against a stencil candidate
 
 
-def workload(name):
+WORKLOAD_PRECISION = {"water": "fp64", "salt": "fp64", "water300k": "mixed", "ensemble": "mixed", "serve": "mixed"}
+WORKLOAD_CPU_STEPS = {"water": 100, "salt": 100, "water300k": 10}
+
+
+def workload(name, precision, rbe_p=None):
     from rbmd_amd import make_salt, make_water, salt_config, water_config
 
     if name == "water":
         s = make_water(n_mol=10000, seed=1)
-        r = water_config()
+        r = water_config(precision=precision, **({"rbe_p": rbe_p} if rbe_p else {}))
         desc = ("rigid 3-site water, 10,000 molecules (30,000 atoms), 1.0 g/cm3 (L=66.88 A), rc 10 A, "
                 "alpha 0.09 (beta 0.3/A), RBE P=200, SHAKE, NVT Nose-Hoover 300 K, dt 2 fs")
         dt_fs = 2.0
+    elif name == "water300k":
+        # P = 2000 keeps the per-step RBE error of the 30,000-atom box (it grows
+        # like sqrt(N) at fixed P); at P = 200 the lattice start heats by
+        # ~250 K per step and SHAKE fails within 10 steps, in the CPU oracle too
+        P = rbe_p or 2000
+        s = make_water(n_mol=100000, seed=1)
+        r = water_config(precision=precision, rbe_p=P)
+        desc = ("rigid 3-site water, 100,000 molecules (300,000 atoms), 1.0 g/cm3 (L=144.09 A), rc 10 A, "
+                f"alpha 0.09 (beta 0.3/A), RBE P={P}, SHAKE, NVT Nose-Hoover 300 K, dt 2 fs")
+        dt_fs = 2.0
     elif name == "salt":
         s = make_salt(n_ions=4000, seed=1)
-        r = salt_config()
+        r = salt_config(precision=precision, **({"rbe_p": rbe_p} if rbe_p else {}))
         desc = ("charged-LJ salt, 4,000 ions, rho*=0.5 (L=20 sigma), rc 4 sigma, alpha 0.64 (beta 0.8), "
                 "RBE P=100, NVT Nose-Hoover T*=1.5, dt 0.001 tau")
         dt_fs = None
     else:
         raise SystemExit(f"unknown workload {name}")
     return s, r, desc, dt_fs
+
+
+def dtype_of(precision):
+    return "f64" if precision == "fp64" else "f32 pair scalar and sin/cos, f64 vectors and sums"
 
 
 def stencil_candidates(g):
@@ -95,16 +114,17 @@ def bench_ensemble(args):
     specs = ensemble_specs(args.systems, args.ens_steps, seed=2024, n_max=args.max_atoms)
     print(f"[bench] ensemble: {len(specs)} systems, {sum(s.atoms for s in specs)} atoms, "
           f"{args.ens_steps} steps each", file=sys.stderr, flush=True)
-    plain = run_plain(specs)
+    plain = run_plain(specs)  # the plain loop is FP64 by definition
     print(f"[bench] plain loop {plain['atom_steps_per_s']:.4g} atom-steps/s", file=sys.stderr, flush=True)
-    packed = run_packed(specs, streams=args.streams, chunk=args.chunk)
+    packed = run_packed(specs, streams=args.streams, chunk=args.chunk, precision=args.precision)
     out = {"metric": "atom-steps/s (ensemble, aggregate)", "value": round(packed["atom_steps_per_s"], 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": args.ens_steps, "warmup": 0,
            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64", "data": "synthetic",
+           "dtype": dtype_of(args.precision), "data": "synthetic",
            "config": {"workload": f"{len(specs)} independent systems, half charged-LJ salt, half rigid water, "
                                   f"atoms log-uniform in [600, {args.max_atoms}], {args.ens_steps} steps each",
-                      "streams": args.streams, "chunk": args.chunk},
+                      "streams": args.streams, "chunk": args.chunk, "precision": args.precision,
+                      "plain_loop_precision": "fp64"},
            "plain_loop_atom_steps_per_s": round(plain["atom_steps_per_s"], 1),
            "speedup_vs_plain_loop": round(packed["atom_steps_per_s"] / plain["atom_steps_per_s"], 3)}
     print(json.dumps(out))
@@ -118,14 +138,16 @@ def bench_serve(args):
     jobs = stream_specs(args.rate, args.duration, seed=77, n_max=args.max_atoms, steps_min=200,
                         steps_max=args.max_steps)
     print(f"[bench] serve: {len(jobs)} jobs over {args.duration} s", file=sys.stderr, flush=True)
-    inputs = {j: build(sp) for j, sp in enumerate(jobs)}  # each job's input as it arrives
-    r = serve(jobs, streams=args.streams, chunk=args.chunk, prebuilt=inputs)
+    inputs = {j: build(sp, args.precision) for j, sp in enumerate(jobs)}  # each job's input as it arrives
+    r = serve(jobs, streams=args.streams, chunk=args.chunk, prebuilt=inputs, precision=args.precision)
     out = {"metric": "served jobs: p50/p99 latency, aggregate atom-steps/s", "value": round(r["atom_steps_per_s"], 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype_of(args.precision),
+           "data": "synthetic",
            "config": {"workload": f"Poisson arrivals {args.rate}/s for {args.duration} s, salt or water 50/50, "
                                   f"atoms log-uniform in [1000, {args.max_atoms}], steps uniform in "
-                                  f"[200, {args.max_steps}]", "streams": args.streams, "chunk": args.chunk},
+                                  f"[200, {args.max_steps}]", "streams": args.streams, "chunk": args.chunk,
+                      "precision": args.precision},
            "jobs": r["jobs"], "p50_latency_s": r["p50_latency_s"], "p99_latency_s": r["p99_latency_s"],
            "wall_s": round(r["wall_s"], 3)}
     print(json.dumps(out))
@@ -136,7 +158,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="water", choices=["water", "salt", "ensemble", "serve"])
+    ap.add_argument("--workload", default="water", choices=["water", "salt", "water300k", "ensemble", "serve"])
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="default: fp64 for water and salt, mixed for water300k, ensemble and serve")
     ap.add_argument("--systems", type=int, default=64, help="ensemble: number of systems (half salt, half water)")
     ap.add_argument("--ens-steps", type=int, default=200, help="ensemble: MD steps per system")
     ap.add_argument("--streams", type=int, default=8, help="ensemble / serve: contexts (HIP streams)")
@@ -145,7 +169,10 @@ def main():
     ap.add_argument("--duration", type=float, default=5.0, help="serve: seconds of arrivals")
     ap.add_argument("--max-atoms", type=int, default=30000, help="ensemble / serve: largest system")
     ap.add_argument("--max-steps", type=int, default=500, help="serve: most steps of a job")
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=None, help="default: 100 (10 for water300k)")
+    ap.add_argument("--rbe-p", type=int, default=None, help="RBE k vectors per step (default: the workload's)")
+    ap.add_argument("--error-sets", type=int, default=16,
+                    help="k sets averaged in the RBE-vs-Ewald force error (0: skip the error report)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--graph", action="store_true", help="replay a captured step graph (rbmd_md_run_graph)")
@@ -170,11 +197,17 @@ def main():
 
     from rbmd_amd.lib import Context, GpuSystem
 
+    if args.precision is None:
+        args.precision = WORKLOAD_PRECISION[args.workload]
+    if args.cpu_steps is None:
+        args.cpu_steps = WORKLOAD_CPU_STEPS.get(args.workload, 100)
     if args.workload == "ensemble":
         return bench_ensemble(args)
     if args.workload == "serve":
         return bench_serve(args)
-    system, run, desc, dt_fs = workload(args.workload)
+    system, run, desc, dt_fs = workload(args.workload, args.precision, args.rbe_p)
+    if args.rbe_p and args.workload != "water300k":
+        desc = desc.replace("RBE P=200", f"RBE P={args.rbe_p}").replace("RBE P=100", f"RBE P={args.rbe_p}")
     print(f"[bench] workload {args.workload}: {system.n_atoms} atoms", file=sys.stderr, flush=True)
     ctx = Context(0)
     g = GpuSystem(ctx, system, run)
@@ -211,7 +244,7 @@ def main():
     }
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tpath) and args.workload == "water":  # measured on the water workload only
+    if os.path.exists(tpath) and args.workload == "water" and args.precision == "fp64":  # measured there only
         with open(tpath) as f:
             tt = json.load(f).get("kernels", {}).get(dom)
         if tt:
@@ -219,9 +252,14 @@ def main():
     roof = None
     if dom in flops:
         achieved = flops[dom] / (kernels[dom]["mean_us"] * 1e-6) / 1e12
-        roof = {"kernel": dom, "bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                "work_per_launch": flops[dom], "peak_note": "MI355X FP64 vector peak (equal to FP64 matrix peak)"}
+        if args.precision == "mixed" and dom == "pair":
+            peak, note = FP32_PEAK_TFLOPS, ("MI355X FP32 vector peak (counts packed FP32, which this build does not "
+                                            "emit; the pair scalar is FP32, its vectors and sums FP64)")
+        else:
+            peak, note = FP64_PEAK_TFLOPS, "MI355X FP64 vector peak (equal to FP64 matrix peak)"
+        roof = {"kernel": dom, "bound": "valu", "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+                "work_per_launch": flops[dom], "peak_note": note}
     th = g.thermo()
     out = {
         "metric": "atom-steps/s",
@@ -234,9 +272,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype_of(args.precision),
         "data": "synthetic",
-        "config": {"workload": desc, "atoms": system.n_atoms, "precision": "fp64", "rbe_p": P,
+        "config": {"workload": desc, "atoms": system.n_atoms, "precision": args.precision, "rbe_p": P,
                    "cut_off": run.cut_off},
         "ns_per_day": round((args.steps / elapsed) * dt_fs * 86400 / 1e6, 3) if dt_fs else None,
         "roofline": roof,
@@ -247,6 +285,27 @@ def main():
         "final_temperature": th["temperature"],
         "status_flags": th["status_flags"],
     }
+    if args.error_sets > 0:
+        from rbmd_amd.accuracy import precision_error, rbe_method_error
+
+        # errors on the configuration the timed run ended on
+        x, v, _, _ = g.state()
+        cur = type(system)(**{**system.__dict__, "x": x, "v": v})
+        fe = {}
+        if run.coulomb_type == "RBE":
+            me = rbe_method_error(ctx, cur, run, n_sets=args.error_sets)
+            fe.update({"rbe_vs_ewald_per_step": round(me["per_step_rel_total"], 5),
+                       f"rbe_vs_ewald_avg{args.error_sets}": round(me["averaged_rel_total"], 5),
+                       "ewald_kmax": me["kmax"]})
+        if args.precision == "mixed":
+            pe = precision_error(ctx, cur, run)
+            fe.update({"mixed_vs_fp64_total": pe.get("total_rel_rms"), "mixed_vs_fp64_real": pe["real_rel_rms"],
+                       "mixed_vs_fp64_rbe": pe.get("rbe_rel_rms"), "mixed_vs_fp64_e_lj": pe["e_lj_rel"],
+                       "mixed_vs_fp64_e_coul": pe["e_coul_rel"]})
+        fe["definition"] = ("relative RMS over atoms against the FP64 reference's nonbonded force "
+                            "(real-space LJ+erfc + k space); RBE vs converged FP64 Ewald at the same alpha, "
+                            "per step (one k set) and averaged over k sets")
+        out["force_error"] = fe
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu_value, cpu_time = cpu_baseline(system, run, args.cpu_steps, threads)

@@ -59,9 +59,13 @@ enum { RBMD_SHAKE_NULL = 0, RBMD_SHAKE_TRUE = 1, RBMD_SHAKE_FALSE = 2 };
  * framework/run/ExecutionNVE.cpp:36-41). */
 enum { RBMD_TEMP_NOSE_HOOVER = 0, RBMD_TEMP_BERENDSEN = 1, RBMD_TEMP_RESCALE = 2, RBMD_TEMP_NONE = 3 };
 /* Arithmetic of the pair / structure-factor kernels.  FP64: everything in
- * double.  MIXED: pair and rho(k) math in FP32 with compensated accumulation;
- * positions, velocities, SHAKE and global reductions stay FP64 (not built yet:
- * rbmd_sys_create returns RBMD_ERR_UNSUPPORTED). */
+ * double.  MIXED: the pair scalar (1/r, LJ, the erfc/exp table) and the
+ * sin/cos of every rho(k) and k-space force term in FP32, each FP32 result
+ * summed in FP64 (exact per term, no worse than two-sum compensation); pair
+ * vectors, phases, positions, velocities, SHAKE and all reductions stay FP64.
+ * Built for the VERLET list path (pair_path LISTS, n_types <= 16); other
+ * combinations return RBMD_ERR_UNSUPPORTED.  rbmd_ewald_force is FP64 in
+ * both. */
 enum { RBMD_PREC_FP64 = 0, RBMD_PREC_MIXED = 1 };
 /* Real-space erfc kernel: analytic G(r) (ExecForceFunction.h:682-689) or the
  * fixed alpha=0.1 step table of the RBL path (ExecStaticTable.h:96-135). */

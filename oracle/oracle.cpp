@@ -196,9 +196,15 @@ void build_locator(orc_sys* s) {
   for (int i = 0; i < n; ++i) s->point_ids[cur[s->cell_of[i]]++] = i;
 }
 
-// ExecPointLocator::InWhichCell (unclamped), ExecPointLocator.h:271-274
+// ExecPointLocator::InWhichCell (unclamped), ExecPointLocator.h:271-274.  A
+// NaN coordinate maps to cell 0 and the index is held within +-2e9 so the
+// integer conversion is defined (both only reachable after a failed step).
 inline void in_which_cell(const orc_sys* s, V3 p, int ijk[3]) {
-  for (int d = 0; d < 3; ++d) ijk[d] = trunc_id((at(p, d) - s->lo[d]) / s->dxdydz[d]);
+  for (int d = 0; d < 3; ++d) {
+    double t = (at(p, d) - s->lo[d]) / s->dxdydz[d];
+    if (!(t == t)) t = 0.0;
+    ijk[d] = trunc_id(std::max(std::min(t, 2.0e9), -2.0e9));
+  }
 }
 
 // Visit the (2nc+1)^3 stencil of atom i in the reference's order
@@ -215,10 +221,15 @@ void visit_stencil(const orc_sys* s, int i, F&& fn) {
         int nb[3] = {ci[0] + a, ci[1] + b, ci[2] + c};
         int w[3];
         V3 off;
+        bool inside = true;
         for (int d = 0; d < 3; ++d) {
           w[d] = (nb[d] + s->dims[d]) % s->dims[d];  // PeriodicIndexOffset
           at(off, d) = s->dxdydz[d] * (double)(w[d] - nb[d]);  // PeriodicCoordOffset
+          // an atom more than one box away wraps to no cell (the reference
+          // would index out of range); only reachable after a failed step
+          inside = inside && w[d] >= 0;
         }
+        if (!inside) continue;
         int cell = w[0] + w[1] * s->dims[0] + w[2] * s->dims[0] * s->dims[1];
         for (int pidx = s->lower[cell]; pidx < s->upper[cell]; ++pidx) {
           int j = s->point_ids[pidx];
@@ -1621,7 +1632,12 @@ int orc_md_step(orc_sys* s, int64_t nsteps) {
       for (int i = 0; i < s->n; ++i) s->v[i] = coeff * s->v[i];
       compute_pressure(s);
     }
+    // a failed constraint or a non-finite position ends the run
+    for (int i = 0; i < s->n && !(s->status_flags & 1); ++i)
+      if (!std::isfinite(s->x[i].x) || !std::isfinite(s->x[i].y) || !std::isfinite(s->x[i].z)) s->status_flags |= 1;
+    if (s->status_flags & 3) break;
   }
+  if (s->status_flags & 1) return RBMD_ERR_NONFINITE;
   return (s->status_flags & 2) ? RBMD_ERR_SHAKE : ((s->status_flags & 4) ? RBMD_ERR_CAPACITY : RBMD_OK);
 }
 

@@ -258,13 +258,14 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // Piecewise polynomial table of the Coulomb radial factors over [0, rc]:
 // interval k = [k h, (k+1) h], local variable u = r/h - k - 1/2 in [-1/2, 1/2].
-// Per interval the degree-kCoulD Chebyshev interpolant at the Chebyshev nodes
-// (coefficients by the discrete cosine sums, then converted to powers of u),
-// all in long double.
+// Per interval the degree-`degree` (odd) Chebyshev interpolant at the
+// Chebyshev nodes (coefficients by the discrete cosine sums, then converted to
+// powers of u), all in long double.
 // P(r) = erfc(beta r) + 2 beta/sqrt(pi) r exp(-beta^2 r^2),  Q(r) = erfc(beta r).
-// Layout (doubles): [f][d / 2][k][d % 2], f = 0 P, 1 Q.
-std::vector<double> coul_table(double beta, double rc) {
-  const int M = kCoulM, K = kCoulD + 1;
+// Layout: [f][d / 2][k][d % 2], f = 0 P, 1 Q.  The FP64 path uses degree
+// kCoulD; the mixed-precision path rounds a degree-kCoulD32 table to FP32.
+std::vector<double> coul_table(double beta, double rc, int degree) {
+  const int M = kCoulM, K = degree + 1;
   const long double pi = 3.14159265358979323846264338327950288L;
   const long double h = (long double)rc / M;
   const long double two_over_sqrtpi = 1.1283791670955125738961589031215452L;
@@ -278,7 +279,7 @@ std::vector<double> coul_table(double beta, double rc) {
       tcoef[n][d] = (d > 0 ? 2 * tcoef[n - 1][d - 1] : 0.0L) - tcoef[n - 2][d];
   for (int k = 0; k < M; ++k) {
     for (int f = 0; f < 2; ++f) {
-      long double y[K], x[K];
+      std::vector<long double> y(K), x(K);
       for (int j = 0; j < K; ++j) {
         x[j] = cosl(pi * (j + 0.5L) / K);  // node in [-1, 1]; u = x / 2
         const long double r = (k + 0.5L + 0.5L * x[j]) * h;
@@ -286,8 +287,7 @@ std::vector<double> coul_table(double beta, double rc) {
         y[j] = erfcl(br);
         if (f == 0) y[j] += two_over_sqrtpi * (long double)beta * r * expl(-br * br);
       }
-      long double mono[K];
-      for (int d = 0; d < K; ++d) mono[d] = 0;
+      std::vector<long double> mono(K, 0.0L);
       for (int nn = 0; nn < K; ++nn) {
         long double a = 0;
         for (int j = 0; j < K; ++j) a += y[j] * cosl(pi * nn * (j + 0.5L) / K);
@@ -463,6 +463,7 @@ void compute_force(rbmd_sys* s, const double* kvec, int thermo) {
     k.P = s->P;
     k.kvec = kvec;
     k.S_over_P = s->p.coulomb_type == RBMD_COULOMB_RBE ? s->S / (double)s->P : 1.0;
+    k.f32 = s->v.mixed;
     int ewald = s->p.coulomb_type == RBMD_COULOMB_EWALD;
     c->run("rho_partial", [&] { launch_rho_partial(s->v, k, c->stream); });
     c->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, c->stream); });
@@ -661,7 +662,14 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   if (!d->type || !d->charge || !d->mass || !d->eps || !d->sigma) return fail(RBMD_ERR_ARG, "missing per-atom arrays");
   if (d->n_atoms >= (1 << (31 - kShiftBits))) return fail(RBMD_ERR_UNSUPPORTED, "n_atoms above %d", 1 << (31 - kShiftBits));
   if (p->cut_off <= 0) return fail(RBMD_ERR_ARG, "cut_off must be positive");
-  if (p->precision != RBMD_PREC_FP64) return fail(RBMD_ERR_UNSUPPORTED, "only RBMD_PREC_FP64 is built");
+  if (p->precision != RBMD_PREC_FP64 && p->precision != RBMD_PREC_MIXED) return fail(RBMD_ERR_ARG, "bad precision");
+  if (p->precision == RBMD_PREC_MIXED) {
+    if (p->neighbor_type == RBMD_NEIGHBOR_RBL)
+      return fail(RBMD_ERR_UNSUPPORTED, "mixed precision is built for the Verlet-list path only");
+    if (p->pair_path == RBMD_PAIR_CELLS)
+      return fail(RBMD_ERR_UNSUPPORTED, "mixed precision is built for pair_path lists only");
+    if (d->n_types > kMaxTypes) return fail(RBMD_ERR_UNSUPPORTED, "mixed precision needs n_types <= %d", kMaxTypes);
+  }
   for (int k = 0; k < 3; ++k)
     if (!(d->hi[k] > d->lo[k])) return fail(RBMD_ERR_ARG, "empty box along axis %d", k);
   for (int i = 0; i < d->n_atoms; ++i) {
@@ -707,6 +715,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   v.lj_only = p->force_field == RBMD_FF_LJ_CUT;
   v.rbl = p->neighbor_type == RBMD_NEIGHBOR_RBL;
   v.erfc_table = p->erfc_mode == RBMD_ERFC_TABLE;
+  v.mixed = p->precision == RBMD_PREC_MIXED ? 1 : 0;
   // accepted pairs have |r| < rc, so the minimum image is a no-op unless an edge is below 2 rc
   v.use_min_image = (v.box[0] < 2 * v.cut || v.box[1] < 2 * v.cut || v.box[2] < 2 * v.cut) ? 1 : 0;
   // the fused kernel keeps the type-pair tables in LDS; more types take the list path
@@ -759,11 +768,19 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   A(s->upload(&v.eps_ij, eij.data(), eij.size()));
   A(s->upload(&v.sig6_ij, s6.data(), s6.size()));
   {
-    std::vector<double> ct = coul_table(v.sqrt_alpha > 0 ? v.sqrt_alpha : 0.0, p->cut_off);
+    const double beta = v.sqrt_alpha > 0 ? v.sqrt_alpha : 0.0;
+    std::vector<double> ct = coul_table(beta, p->cut_off, kCoulD);
     double* dct = nullptr;
     A(s->upload(&dct, ct.data(), ct.size()));
     v.ctab = reinterpret_cast<const double2*>(dct);
     v.ctab_inv_h = kCoulM / p->cut_off;
+    if (v.mixed) {
+      std::vector<double> c32 = coul_table(beta, p->cut_off, kCoulD32);
+      std::vector<float> f32(c32.begin(), c32.end());
+      float* dft = nullptr;
+      A(s->upload(&dft, f32.data(), f32.size()));
+      v.ctab32 = reinterpret_cast<const float2*>(dft);
+    }
   }
   if (have_sp) {
     int ns = d->special_offsets[n];
@@ -1281,6 +1298,7 @@ rbmd_status rbmd_rbe_structure_factor(rbmd_sys* s, double* rho_out) {
   KView k = s->kv;
   k.P = s->P;
   k.kvec = s->kcur;
+  k.f32 = s->v.mixed;
   hipStream_t st = s->ctx->stream;
   s->ctx->run("rho_partial", [&] { launch_rho_partial(s->v, k, st); });
   s->ctx->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, st); });
@@ -1295,6 +1313,7 @@ static rbmd_status kforce_out(rbmd_sys* s, int ewald, double* force, double* vir
   k.P = s->P;
   k.kvec = s->kcur;
   k.S_over_P = ewald ? 1.0 : s->S / (double)s->P;
+  k.f32 = ewald ? 0 : s->v.mixed;  // the converged Ewald sum is always FP64
   hipStream_t st = s->ctx->stream;
   s->ctx->run("rho_partial", [&] { launch_rho_partial(s->v, k, st); });
   s->ctx->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, st); });

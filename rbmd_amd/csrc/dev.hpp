@@ -17,6 +17,9 @@ constexpr int kCoulM = 16;      // Coulomb table intervals over [0, rc]
 constexpr int kCoulD = 11;      // polynomial degree per interval
 constexpr int kCoulPairs = (kCoulD + 1) / 2;           // coefficient pairs per interval
 constexpr int kCoulTab = 2 * kCoulPairs * kCoulM;      // double2 entries (P and Q)
+constexpr int kCoulD32 = 5;     // degree of the FP32 table of the mixed-precision path
+constexpr int kCoulPairs32 = (kCoulD32 + 1) / 2;
+constexpr int kCoulTab32 = 2 * kCoulPairs32 * kCoulM;  // float2 entries
 constexpr int kMaxTypes = 16;   // type-pair tables held in LDS up to this many types
 
 // Neighbour-list slot layout: one row of `cap` slots per atom (sorted order),
@@ -43,6 +46,7 @@ struct SysView {
   int rs_num, pice_num;
   int nslab;        // real-space partials per atom in fn*, vir_near, e_lj, e_coul (fused path 3, else 1)
   int use_min_image;  // some box edge < 2 rc: the pair vector may need the minimum image
+  int mixed;        // RBMD_PREC_MIXED: the pair scalar in FP32 (pair vectors and sums stay FP64)
   double lo[3], hi[3], box[3], dxdydz[3];
   double volume, cut, cut2, rs, rs2, alpha, sqrt_alpha, qqr2e;
   // per atom, original order (SoA)
@@ -60,6 +64,7 @@ struct SysView {
   // interval k, so lanes at different distances read different LDS banks
   const double2* ctab;  // [2][kCoulPairs][kCoulM]
   double ctab_inv_h;
+  const float2* ctab32; // the same functions at degree kCoulD32 in FP32: [2][kCoulPairs32][kCoulM]
   // special lists (CSR) and molecule groups
   int *sp_off, *sp_ids;
   double *sp_w;
@@ -152,6 +157,7 @@ struct KView {
   double* rho_part;     // [nblk_rho][2P]
   double* rho;          // [2P]
   int nblk_rho;
+  int f32;              // phases' sin/cos and the force coefficient in FP32 (sums in FP64)
 };
 void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st);
 void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st);

@@ -13,27 +13,41 @@
 //   the same kernel is ComputeNewFarElectrostaticsWorklet (RunWorklet.cpp:2492-2540).
 #include "dev.hpp"
 #include "reduce.hpp"
+#include "sincos.hpp"
 
 namespace rbmd {
 
 constexpr int kRhoAtomsPerThread = 4;
 constexpr int kRhoK = 8;  // k-vectors per workgroup in the rho pass
 
-// e^{i k.r} with k = 2 pi m / L: the phase m.(r/L) in turns is reduced to
-// [-1/2, 1/2] before sincospi, so no large-argument reduction is needed
-// (k.r reaches several hundred radians at |m| ~ 30).
+// e^{i k.r} with k = 2 pi m / L is e^{2 pi i t} with the phase t = m.(r/L) in
+// turns (several hundred radians at |m| ~ 30): sincos_turns reduces it
+// exactly to a quadrant and a remainder of at most 1/8 turn before its
+// polynomials, so no large-argument reduction is needed.  F32 (mixed
+// precision): the remainder's polynomials in FP32 (|error| <= 1.1e-7); the
+// phase, the reduction and every sum stay FP64.
+template <bool F32>
 __device__ __forceinline__ void phase_sincos(double mx, double my, double mz, double sx, double sy, double sz,
                                              double* sn, double* cs) {
-  double t = mx * sx + my * sy + mz * sz;
-  t -= rint(t);
-  sincospi(2.0 * t, sn, cs);
+  const double t = mx * sx + my * sy + mz * sz;
+  if constexpr (F32) {
+    float s32, c32;
+    sincos_turns32(t, &s32, &c32);
+    *sn = s32;
+    *cs = c32;
+  } else {
+    sincos_turns(t, sn, cs);
+  }
 }
 
-__device__ __forceinline__ double k_to_m(double k, double box) { return rint(k * box / (2.0 * kPi)); }
+// m = k L / (2 pi), an integer held in k to ~1e-13: the product with the
+// reciprocal rounds to the same integer as the quotient
+__device__ __forceinline__ double k_to_m(double k, double box) { return rint(k * box * (0.5 / kPi)); }
 
+template <bool F32>
 __global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
   __shared__ double lds[kBlock / kWave][2 * kRhoK];
-  const int l0 = blockIdx.y * kRhoK;
+  const int l0 = blockIdx.x * kRhoK;  // k-vector block on x (a converged Ewald grid has millions)
   double re[kRhoK], im[kRhoK];
 #pragma unroll
   for (int t = 0; t < kRhoK; ++t) re[t] = im[t] = 0.0;
@@ -46,16 +60,17 @@ __global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
     my[t] = ok ? k_to_m(kv.kvec[3 * l + 1], s.box[1]) : 0.0;
     mz[t] = ok ? k_to_m(kv.kvec[3 * l + 2], s.box[2]) : 0.0;
   }
-  const int base = blockIdx.x * kBlock * kRhoAtomsPerThread;
+  const int base = blockIdx.y * kBlock * kRhoAtomsPerThread;
+  const double ibx = 1.0 / s.box[0], iby = 1.0 / s.box[1], ibz = 1.0 / s.box[2];
 #pragma unroll
   for (int a = 0; a < kRhoAtomsPerThread; ++a) {
     int i = base + a * kBlock + threadIdx.x;
     if (i < s.n) {
-      double sx = s.x[i] / s.box[0], sy = s.y[i] / s.box[1], sz = s.z[i] / s.box[2], q = s.q[i];
+      double sx = s.x[i] * ibx, sy = s.y[i] * iby, sz = s.z[i] * ibz, q = s.q[i];
 #pragma unroll
       for (int t = 0; t < kRhoK; ++t) {
         double sn, cs;
-        phase_sincos(mx[t], my[t], mz[t], sx, sy, sz, &sn, &cs);
+        phase_sincos<F32>(mx[t], my[t], mz[t], sx, sy, sz, &sn, &cs);
         re[t] += q * cs;
         im[t] += q * sn;
       }
@@ -79,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
     if (l < kv.P) {
       double acc = 0.0;
       for (int w = 0; w < kBlock / kWave; ++w) acc += lds[w][threadIdx.x];
-      kv.rho_part[(size_t)blockIdx.x * 2 * kv.P + 2 * l + (threadIdx.x & 1)] = acc;
+      kv.rho_part[(size_t)blockIdx.y * 2 * kv.P + 2 * l + (threadIdx.x & 1)] = acc;
     }
   }
 }
@@ -107,8 +122,11 @@ void launch_kvec_slot(double* dst, const double* ring, int R, int per, const int
 }
 
 void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st) {
-  dim3 grid(k.nblk_rho, (k.P + kRhoK - 1) / kRhoK);
-  hipLaunchKernelGGL(k_rho_partial, grid, dim3(kBlock), 0, st, s, k);
+  dim3 grid((k.P + kRhoK - 1) / kRhoK, k.nblk_rho);
+  if (k.f32)
+    hipLaunchKernelGGL(k_rho_partial<true>, grid, dim3(kBlock), 0, st, s, k);
+  else
+    hipLaunchKernelGGL(k_rho_partial<false>, grid, dim3(kBlock), 0, st, s, k);
 }
 
 void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st) {
@@ -120,7 +138,7 @@ constexpr int kKLg = 2;  // 4 lanes per atom split the k list
 
 // prefactor pre_l = -4 pi / (V k^2) [* e^{-k^2/4alpha} for Ewald]; lane `sub`
 // of an atom takes k-vectors sub, sub + 4, ... of each LDS tile
-template <bool THERMO, bool PERATOM>
+template <bool THERMO, bool PERATOM, bool F32>
 __global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewald, double alpha, double* rows) {
   constexpr int L = 1 << kKLg;
   __shared__ double tk[kForceTile][4];
@@ -136,9 +154,9 @@ __global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewal
     y = s.y[i];
     z = s.z[i];
     q = s.q[i];
-    sx = x / s.box[0];
-    sy = y / s.box[1];
-    sz = z / s.box[2];
+    sx = x * (1.0 / s.box[0]);
+    sy = y * (1.0 / s.box[1]);
+    sz = z * (1.0 / s.box[2]);
   }
   double fx = 0, fy = 0, fz = 0;
   double v[6] = {0, 0, 0, 0, 0, 0};
@@ -165,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewal
     for (int u = sub; u < cnt; u += L) {
       double kx = tk[u][0], ky = tk[u][1], kz = tk[u][2];
       double sn, cs;
-      phase_sincos(tm[u][0], tm[u][1], tm[u][2], sx, sy, sz, &sn, &cs);
+      phase_sincos<F32>(tm[u][0], tm[u][1], tm[u][2], sx, sy, sz, &sn, &cs);
       double fs = (-4.0 * kPi * q) * tk[u][3] * (cs * tr[u][1] - sn * tr[u][0]);
       fx += fs * kx;
       fy += fs * ky;
@@ -208,12 +226,21 @@ void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, 
                    double* rows, int* nblocks) {
   int grid = kforce_blocks(s);
   if (nblocks) *nblocks = grid;
+  if (k.f32) {
+    if (peratom)
+      hipLaunchKernelGGL((k_kforce<true, true, true>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    else if (thermo)
+      hipLaunchKernelGGL((k_kforce<true, false, true>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    else
+      hipLaunchKernelGGL((k_kforce<false, false, true>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    return;
+  }
   if (peratom)
-    hipLaunchKernelGGL((k_kforce<true, true>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    hipLaunchKernelGGL((k_kforce<true, true, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
   else if (thermo)
-    hipLaunchKernelGGL((k_kforce<true, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    hipLaunchKernelGGL((k_kforce<true, false, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
   else
-    hipLaunchKernelGGL((k_kforce<false, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    hipLaunchKernelGGL((k_kforce<false, false, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
 }
 
 }  // namespace rbmd

@@ -7,7 +7,7 @@
 // registers; the L partials are combined with a fixed xor-shuffle tree.
 // Forces are gathered, never scattered: no atomics, deterministic sums.
 //
-// Formulas (all FP64 here):
+// Formulas (FP64; the mixed-precision variant evaluates the pair scalar in FP32):
 //   VERLET plain    ComputeNearForceVerletVirialWorklet, RunWorklet.cpp:2890-2985
 //   VERLET weighted ComputeNearForceVerletWeightVirialWorklet1, RunWorklet.cpp:2767-2888
 //   LJ/CUT only     ComputeLJForceVerletWorklet, RunWorklet.cpp:498-575
@@ -15,6 +15,8 @@
 //   LJ      ExecForceFunction.h:412-435 (force), :818-841 (energy)
 //   G(r)    ExecForceFunction.h:682-689, erfc(sqrt(alpha) r)/r^2 + 2 sqrt(alpha/pi) e^{-alpha r^2}/r
 //   virial  ComputePairVirial_fix (+1/2, plain) / ComputePairVirial (-1/2, weighted, RBL), :304-332
+#include <type_traits>
+
 #include "dev.hpp"
 #include "reduce.hpp"
 #include "pair.hpp"
@@ -59,20 +61,33 @@ __device__ __forceinline__ int cursor_take(ListCursor& c, const int (&cnt)[kMaxS
 // type-pair tables in LDS (ntypes <= kMaxTypes).  Two entries per lane per
 // iteration; the neighbour positions of the next iteration are gathered
 // while the current one is evaluated, and its entries one iteration earlier.
-template <int MODE, bool THERMO, bool PERATOM, int SP, bool LTAB>
+// F32 (mixed precision): the pair scalar (1/r, LJ, the Coulomb table) in FP32
+// from an FP32 table in LDS; the pair vector, d2, the force and virial sums
+// and the energies stay FP64 (each FP32 term is exact in FP64, so the sums
+// carry no more rounding than the FP64 path's).
+template <int MODE, bool THERMO, bool PERATOM, int SP, bool LTAB, bool F32>
 __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows) {
   constexpr bool EN = THERMO || PERATOM;
   constexpr int SPD = SP > 0 ? SP : 1;
-  __shared__ double teps[LTAB ? kMaxTypes * kMaxTypes : 1], tsig[LTAB ? kMaxTypes * kMaxTypes : 1];
-  __shared__ double2 ctab[MODE == 2 ? 1 : kCoulTab];
+  using TT = typename std::conditional<F32, float, double>::type;
+  using CT = typename std::conditional<F32, float2, double2>::type;
+  constexpr int kTab = F32 ? kCoulTab32 : kCoulTab;
+  __shared__ TT teps[LTAB ? kMaxTypes * kMaxTypes : 1], tsig[LTAB ? kMaxTypes * kMaxTypes : 1];
+  __shared__ CT ctab[MODE == 2 ? 1 : kTab];
   const int nt = s.ntypes;
   if (LTAB)
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
-      teps[k] = s.eps_ij[k];
-      tsig[k] = s.sig6_ij[k];
+      teps[k] = (TT)s.eps_ij[k];
+      tsig[k] = (TT)s.sig6_ij[k];
     }
-  if (MODE != 2)
-    for (int k = threadIdx.x; k < kCoulTab; k += blockDim.x) ctab[k] = s.ctab[k];
+  if (MODE != 2) {
+    const CT* src;
+    if constexpr (F32)
+      src = s.ctab32;
+    else
+      src = s.ctab;
+    for (int k = threadIdx.x; k < kTab; k += blockDim.x) ctab[k] = src[k];
+  }
   __syncthreads();
   const int lg = s.lanes_lg;
   const int L = 1 << lg;
@@ -139,25 +154,34 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
         if (s.use_min_image) min_image(s, rx, ry, rz);
         const double d2 = rx * rx + ry * ry + rz * rz;
         const int tix = ti * nt + tj;
-        const double epsij = LTAB ? teps[tix] : s.eps_ij[tix];
-        const double sig6 = LTAB ? tsig[tix] : s.sig6_ij[tix];
+        const TT epsij = LTAB ? teps[tix] : (TT)s.eps_ij[tix];
+        const TT sig6 = LTAB ? tsig[tix] : (TT)s.sig6_ij[tix];
         double w = 1.0;
         if (MODE == 1) {
 #pragma unroll
           for (int u = 0; u < SPD; ++u)
             if (spj[u] == j) w = spw[u];
         }
-        double elj, ec;
+        double sc, elj, ec;
         // list entries satisfy the acceptance test with this same vector;
         // with the minimum image applied the range is tested again
-        const double sc = s.use_min_image ? pair_term<MODE, EN, true>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec)
-                                          : pair_term<MODE, EN, false>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec);
+        if constexpr (F32) {
+          float el32, ec32;
+          const float qq = (float)(pi.w * p.w);
+          sc = s.use_min_image ? pair_term32<MODE, EN, true>(s, ctab, d2, qq, epsij, sig6, (float)w, el32, ec32)
+                               : pair_term32<MODE, EN, false>(s, ctab, d2, qq, epsij, sig6, (float)w, el32, ec32);
+          elj = el32;
+          ec = ec32;
+        } else {
+          sc = s.use_min_image ? pair_term<MODE, EN, true>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec)
+                               : pair_term<MODE, EN, false>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec);
+        }
         const double px = sc * rx, py = sc * ry, pz = sc * rz;
         fx += px;
         fy += py;
         fz += pz;
         if (EN) {
-          vir_add(vir, rx, ry, rz, px, py, pz, MODE == 1 ? -0.5 : 0.5);
+          vir_add(vir, rx, ry, rz, px, py, pz, 1.0);  // the +-1/2 is applied once at the end
           pelj += elj;
           pec += ec;
         }
@@ -171,6 +195,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
       na = ma;
       nb = mb;
     }
+  }
+  if (EN) {
+    // a power-of-two scale commutes exactly with the sums
+    const double vs = MODE == 1 ? -0.5 : 0.5;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) vir[k] *= vs;
   }
   if (THERMO) {
     const double vals[kRowPair] = {pelj, pec, vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
@@ -198,22 +228,27 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
   }
 }
 
-template <int MODE, int SP, bool LTAB>
+template <int MODE, int SP, bool LTAB, bool F32>
 void launch_pair_v(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int grid) {
   if (peratom)
-    hipLaunchKernelGGL((k_pair_verlet<MODE, true, true, SP, LTAB>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+    hipLaunchKernelGGL((k_pair_verlet<MODE, true, true, SP, LTAB, F32>), dim3(grid), dim3(kBlock), 0, st, s, rows);
   else if (thermo)
-    hipLaunchKernelGGL((k_pair_verlet<MODE, true, false, SP, LTAB>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+    hipLaunchKernelGGL((k_pair_verlet<MODE, true, false, SP, LTAB, F32>), dim3(grid), dim3(kBlock), 0, st, s, rows);
   else
-    hipLaunchKernelGGL((k_pair_verlet<MODE, false, false, SP, LTAB>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+    hipLaunchKernelGGL((k_pair_verlet<MODE, false, false, SP, LTAB, F32>), dim3(grid), dim3(kBlock), 0, st, s,
+                       rows);
 }
 
+// the mixed-precision kernels are built with LDS type tables only (the host
+// refuses mixed precision with more than kMaxTypes types)
 template <int MODE, int SP>
 void launch_pair_t(const SysView& s, hipStream_t st, int thermo, int peratom, double* rows, int grid) {
-  if (s.ntypes <= kMaxTypes)
-    launch_pair_v<MODE, SP, true>(s, st, thermo, peratom, rows, grid);
+  if (s.mixed)
+    launch_pair_v<MODE, SP, true, true>(s, st, thermo, peratom, rows, grid);
+  else if (s.ntypes <= kMaxTypes)
+    launch_pair_v<MODE, SP, true, false>(s, st, thermo, peratom, rows, grid);
   else
-    launch_pair_v<MODE, SP, false>(s, st, thermo, peratom, rows, grid);
+    launch_pair_v<MODE, SP, false, false>(s, st, thermo, peratom, rows, grid);
 }
 
 // RBL: per-atom corrected force before the mean removal; the virial keeps the

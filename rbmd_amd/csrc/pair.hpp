@@ -115,4 +115,64 @@ __device__ __forceinline__ double pair_term(const SysView& s, const double2* cta
   return fc + flj;
 }
 
+// FP32 Coulomb table polynomial (degree kCoulD32), same layout as coul_poly
+template <bool ISQ>
+__device__ __forceinline__ float coul_poly32(const float2* ctab, int k, float u) {
+  const float2* c = ctab + (ISQ ? kCoulPairs32 * kCoulM : 0) + k;
+  const float2 top = c[(kCoulPairs32 - 1) * kCoulM];
+  float p = fmaf(top.y, u, top.x);
+#pragma unroll
+  for (int dp = kCoulPairs32 - 2; dp >= 0; --dp) {
+    const float2 cc = c[dp * kCoulM];
+    p = fmaf(p, u, cc.y);
+    p = fmaf(p, u, cc.x);
+  }
+  return p;
+}
+
+// pair_term with the scalar arithmetic in FP32 (mixed precision): the caller
+// forms the pair vector and d2 in FP64 and scales and sums in FP64.  1/sqrt
+// is the hardware reciprocal square root (v_rsq_f32; a Newton step on it left
+// the force error unchanged at ~3e-7 RMS, so none is taken).  The Coulomb
+// factors come from the degree-kCoulD32 FP32 table.
+template <int MODE, bool EN, bool CHECK = true>
+__device__ __forceinline__ float pair_term32(const SysView& s, const float2* ctab, double d2d, float qq, float epsij,
+                                             float sig6, float w, float& elj, float& ec) {
+  const bool in = !CHECK || (d2d < s.cut2 && d2d > 0.0001);
+  const float d2 = (float)d2d;
+  float flj = 0.f, fc = 0.f;
+  elj = 0.f;
+  ec = 0.f;
+  if (in) {
+    const float rinv = __builtin_amdgcn_rsqf(d2);
+    const float rinv2 = rinv * rinv;
+    const float sd6 = sig6 * (rinv2 * rinv2 * rinv2);
+    flj = -24.0f * epsij * ((2.0f * sd6 - 1.0f) * sd6) * rinv2;
+    if (EN) elj = 0.5f * (4.0f * epsij * (sd6 - 1.0f) * sd6);
+    if (MODE != 2) {
+      const float fk = (d2 * rinv) * (float)s.ctab_inv_h;
+      int k = (int)fk;
+      k = k < kCoulM - 1 ? k : kCoulM - 1;
+      const float u = fk - (float)k - 0.5f;
+      const float P = coul_poly32<false>(ctab, k, u);
+      const float rinv3 = rinv2 * rinv;
+      const float qqc = (float)s.qqr2e * qq;
+      if (MODE == 0) {
+        fc = -qqc * P * rinv3;
+      } else {
+        fc = qqc * rinv3 * ((1.0f - w) - P);
+      }
+      if (EN) {
+        const float Q = coul_poly32<true>(ctab, k, u);
+        ec = MODE == 0 ? 0.5f * qqc * Q * rinv : 0.5f * qqc * rinv * (Q - (1.0f - w));
+      }
+    }
+  }
+  if (MODE == 1) {
+    flj *= w;
+    elj *= w;
+  }
+  return fc + flj;
+}
+
 }  // namespace rbmd

@@ -68,20 +68,20 @@ def stream_specs(rate: float, duration: float, seed: int, n_min: int = 1000, n_m
                            int(rng.integers(steps_min, steps_max + 1)), int(rng.integers(1, 2 ** 31)), t))
 
 
-def build(spec: JobSpec):
+def build(spec: JobSpec, precision: str = "fp64"):
     if spec.kind == "salt":
-        return make_salt(n_ions=spec.atoms, seed=spec.seed), salt_config()
-    return make_water(n_mol=spec.atoms // 3, seed=spec.seed), water_config()
+        return make_salt(n_ions=spec.atoms, seed=spec.seed), salt_config(precision=precision)
+    return make_water(n_mol=spec.atoms // 3, seed=spec.seed), water_config(precision=precision)
 
 
-def run_plain(specs: List[JobSpec], device: int = 0) -> dict:
+def run_plain(specs: List[JobSpec], device: int = 0, precision: str = "fp64") -> dict:
     """One system at a time on one stream, eager steps (rbmd_md_step).  Only
     the MD steps are timed (system set-up and md_init are not)."""
     ctx = Context(device)
     busy = 0.0
     atom_steps = 0
     for sp in specs:
-        s, r = build(sp)
+        s, r = build(sp, precision)
         g = GpuSystem(ctx, s, r)
         g.md_init()
         ctx.synchronize()
@@ -116,14 +116,14 @@ class Pool:
 
 
 def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: int = 0,
-               systems: Optional[list] = None) -> dict:
+               systems: Optional[list] = None, precision: str = "fp64") -> dict:
     """All systems resident at once; every round enqueues `chunk` graph steps
     of every unfinished system on its stream, then waits for the round.  Only
     the stepping is timed."""
     pool = Pool(streams, device)
     live = []
     for sp in specs:
-        s, r = build(sp)
+        s, r = build(sp, precision)
         k = pool.pick(sp.atoms)
         g = GpuSystem(pool.ctxs[k], s, r)
         g.md_init()
@@ -157,7 +157,7 @@ def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: 
 
 def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: float = 200e9, max_live: int = 64,
           device: int = 0, time_scale: float = 1.0, results: Optional[dict] = None,
-          prebuilt: Optional[dict] = None) -> dict:
+          prebuilt: Optional[dict] = None, precision: str = "fp64") -> dict:
     """Continuous batching of a job stream.
 
     Jobs arrive at spec.arrival * time_scale seconds after the start.  A job
@@ -186,7 +186,7 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
         # admission in arrival order while a slot and memory are free
         while waiting and len(live) < max_live:
             j = waiting[0]
-            s, r = prebuilt[j] if prebuilt is not None else build(jobs[j])
+            s, r = prebuilt[j] if prebuilt is not None else build(jobs[j], precision)
             need = GpuSystem.estimate_footprint(s, r)
             if live and used + need > hbm_budget:
                 break

@@ -241,41 +241,40 @@ def make_water(n_mol: int = 10000, density_g_cm3: float = 1.0, seed: int = 1, te
     grid[sc[:, 0], sc[:, 1], sc[:, 2]] = np.arange(n_mol)
     offsets = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1) if (dx, dy, dz) != (0, 0, 0)]
 
-    def too_close(idx):
-        """molecules of idx with an intermolecular contact below min_dist"""
-        pos = np.stack([O, O + h1, O + h2], 1)  # [n,3,3]
-        bad = np.zeros(len(idx), bool)
-        for dx, dy, dz in offsets:
+    def contact(ia, ib):
+        """closest atom distance between molecules ia[k] and ib[k] (minimum image);
+        exactly symmetric in (ia, ib)"""
+        pa = np.stack([O[ia], O[ia] + h1[ia], O[ia] + h2[ia]], 1)
+        pb = np.stack([O[ib], O[ib] + h1[ib], O[ib] + h2[ib]], 1)
+        d = pa[:, :, None, :] - pb[:, None, :, :]
+        d -= L * np.round(d / L)
+        return np.sqrt((d * d).sum(-1)).reshape(len(ia), -1).min(1)
+
+    def flag_pairs(idx, offs):
+        """molecules in a contact below min_dist among the pairs (i, lattice
+        neighbour of i at offset o) for i in idx, o in offs; flags both sides"""
+        bad = np.zeros(n_mol, bool)
+        for dx, dy, dz in offs:
             nb = grid[(sc[idx, 0] + dx) % m, (sc[idx, 1] + dy) % m, (sc[idx, 2] + dz) % m]
-            ok = nb >= 0
-            if not ok.any():
+            ii = np.flatnonzero(nb >= 0)
+            if len(ii) == 0:
                 continue
-            ii = np.flatnonzero(ok)
-            pj = pos[nb[ii]]
-            pi = pos[idx[ii]]
-            d = pi[:, :, None, :] - pj[:, None, :, :]
-            d -= L * np.round(d / L)
-            dmin = np.sqrt((d * d).sum(-1)).reshape(len(ii), -1).min(1)
-            bad[ii[dmin < min_dist]] = True
-        return idx[bad]
+            hit = contact(idx[ii], nb[ii]) < min_dist
+            bad[idx[ii][hit]] = True
+            bad[nb[ii][hit]] = True
+        return np.flatnonzero(bad)
 
-    def with_neighbours(idx):
-        out = [idx]
-        for dx, dy, dz in offsets:
-            nb = grid[(sc[idx, 0] + dx) % m, (sc[idx, 1] + dy) % m, (sc[idx, 2] + dz) % m]
-            out.append(nb[nb >= 0])
-        return np.unique(np.concatenate(out))
-
-    # the contact test is symmetric, so after re-orienting the flagged
-    # molecules only they and their lattice neighbours can change status
-    bad = too_close(np.arange(n_mol))
+    # every lattice-neighbour pair once (half of the offsets); then, after the
+    # flagged molecules are re-oriented, only pairs touching them can change
+    # (the test is symmetric and all other pairs were already clear)
+    bad = flag_pairs(np.arange(n_mol), offsets[len(offsets) // 2:])
     for _ in range(199):
         if len(bad) == 0:
             break
         nh1, nh2 = water_geometry(rng, len(bad))
         h1[bad] = nh1
         h2[bad] = nh2
-        bad = too_close(with_neighbours(bad))
+        bad = flag_pairs(bad, offsets)
     x = np.stack([O, O + h1, O + h2], 1).reshape(-1, 3)
     x = np.mod(x, L)
     n = 3 * n_mol

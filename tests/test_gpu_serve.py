@@ -12,8 +12,8 @@ JOBS = [JobSpec("salt", 600, 40, 11), JobSpec("water", 603, 30, 12), JobSpec("sa
         JobSpec("water", 1500, 35, 14, arrival=0.01)]
 
 
-def alone(ctx, spec):
-    s, r = build(spec)
+def alone(ctx, spec, precision="fp64"):
+    s, r = build(spec, precision)
     g = GpuSystem(ctx, s, r)
     g.md_init()
     g.md_step(spec.steps)
@@ -22,12 +22,13 @@ def alone(ctx, spec):
     return out
 
 
-def test_packed_equals_alone(ctx):
-    print("[case] packed ensemble vs alone", flush=True)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_packed_equals_alone(ctx, precision):
+    print(f"[case] packed ensemble vs alone ({precision})", flush=True)
     systems = []
-    run_packed(JOBS, streams=2, chunk=7, systems=systems)
+    run_packed(JOBS, streams=2, chunk=7, systems=systems, precision=precision)
     for spec, g in zip(JOBS, systems):
-        ref = alone(ctx, spec)
+        ref = alone(ctx, spec, precision)
         got = g.state()
         for a, b in zip(ref, got):
             assert np.array_equal(a, b)
@@ -35,12 +36,13 @@ def test_packed_equals_alone(ctx):
         g.close()
 
 
-def test_served_equals_alone(ctx):
-    print("[case] served stream vs alone", flush=True)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_equals_alone(ctx, precision):
+    print(f"[case] served stream vs alone ({precision})", flush=True)
     results = {}
-    r = serve(JOBS, streams=3, chunk=9, results=results, time_scale=1.0)
+    r = serve(JOBS, streams=3, chunk=9, results=results, time_scale=1.0, precision=precision)
     assert r["jobs"] == len(JOBS) and r["p99_latency_s"] >= r["p50_latency_s"] > 0
     for j, spec in enumerate(JOBS):
-        ref = alone(ctx, spec)
+        ref = alone(ctx, spec, precision)
         for a, b in zip(ref, results[j]):
             assert np.array_equal(a, b)
