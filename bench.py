@@ -116,15 +116,16 @@ def bench_ensemble(args):
           f"{args.ens_steps} steps each", file=sys.stderr, flush=True)
     plain = run_plain(specs)  # the plain loop is FP64 by definition
     print(f"[bench] plain loop {plain['atom_steps_per_s']:.4g} atom-steps/s", file=sys.stderr, flush=True)
-    packed = run_packed(specs, streams=args.streams, chunk=args.chunk, precision=args.precision)
+    packed = run_packed(specs, streams=args.streams, chunk=args.chunk, precision=args.precision, mode=args.ens_mode)
     out = {"metric": "atom-steps/s (ensemble, aggregate)", "value": round(packed["atom_steps_per_s"], 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": args.ens_steps, "warmup": 0,
            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": dtype_of(args.precision), "data": "synthetic",
            "config": {"workload": f"{len(specs)} independent systems, half charged-LJ salt, half rigid water, "
                                   f"atoms log-uniform in [600, {args.max_atoms}], {args.ens_steps} steps each",
-                      "streams": args.streams, "chunk": args.chunk, "precision": args.precision,
-                      "plain_loop_precision": "fp64"},
+                      "mode": args.ens_mode, "streams": args.streams if args.ens_mode == "streams" else None,
+                      "chunk": args.chunk, "precision": args.precision, "plain_loop_precision": "fp64",
+                      "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
            "plain_loop_atom_steps_per_s": round(plain["atom_steps_per_s"], 1),
            "speedup_vs_plain_loop": round(packed["atom_steps_per_s"] / plain["atom_steps_per_s"], 3)}
     print(json.dumps(out))
@@ -139,15 +140,17 @@ def bench_serve(args):
                         steps_max=args.max_steps)
     print(f"[bench] serve: {len(jobs)} jobs over {args.duration} s", file=sys.stderr, flush=True)
     inputs = {j: build(sp, args.precision) for j, sp in enumerate(jobs)}  # each job's input as it arrives
-    r = serve(jobs, streams=args.streams, chunk=args.chunk, prebuilt=inputs, precision=args.precision)
+    r = serve(jobs, streams=args.streams, chunk=args.chunk, prebuilt=inputs, precision=args.precision,
+              mode=args.ens_mode)
     out = {"metric": "served jobs: p50/p99 latency, aggregate atom-steps/s", "value": round(r["atom_steps_per_s"], 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype_of(args.precision),
            "data": "synthetic",
            "config": {"workload": f"Poisson arrivals {args.rate}/s for {args.duration} s, salt or water 50/50, "
                                   f"atoms log-uniform in [1000, {args.max_atoms}], steps uniform in "
-                                  f"[200, {args.max_steps}]", "streams": args.streams, "chunk": args.chunk,
-                      "precision": args.precision},
+                                  f"[200, {args.max_steps}]", "mode": args.ens_mode,
+                      "streams": args.streams if args.ens_mode == "streams" else None, "chunk": args.chunk,
+                      "precision": args.precision, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
            "jobs": r["jobs"], "p50_latency_s": r["p50_latency_s"], "p99_latency_s": r["p99_latency_s"],
            "wall_s": round(r["wall_s"], 3)}
     print(json.dumps(out))
@@ -163,7 +166,10 @@ def main():
                     help="default: fp64 for water and salt, mixed for water300k, ensemble and serve")
     ap.add_argument("--systems", type=int, default=64, help="ensemble: number of systems (half salt, half water)")
     ap.add_argument("--ens-steps", type=int, default=200, help="ensemble: MD steps per system")
-    ap.add_argument("--streams", type=int, default=8, help="ensemble / serve: contexts (HIP streams)")
+    ap.add_argument("--ens-mode", choices=["pack", "streams"], default="pack",
+                    help="ensemble / serve: one pack per kind (one launch per kernel for all its systems) or "
+                         "per-system step graphs spread over --streams contexts")
+    ap.add_argument("--streams", type=int, default=8, help="ensemble / serve with --ens-mode streams: contexts")
     ap.add_argument("--chunk", type=int, default=50, help="ensemble / serve: graph steps per enqueue round")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="serve: seconds of arrivals")
@@ -171,6 +177,8 @@ def main():
     ap.add_argument("--max-steps", type=int, default=500, help="serve: most steps of a job")
     ap.add_argument("--cpu-steps", type=int, default=None, help="default: 100 (10 for water300k)")
     ap.add_argument("--rbe-p", type=int, default=None, help="RBE k vectors per step (default: the workload's)")
+    ap.add_argument("--hw-queues", type=int, default=None,
+                    help="ensemble / serve: hardware queues HIP gives this process (GPU_MAX_HW_QUEUES, 1..32)")
     ap.add_argument("--error-sets", type=int, default=16,
                     help="k sets averaged in the RBE-vs-Ewald force error (0: skip the error report)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -195,6 +203,10 @@ def main():
               file=sys.stderr)
         raise SystemExit(2)
 
+    if args.hw_queues is not None:
+        if not 1 <= args.hw_queues <= 32:
+            raise SystemExit("--hw-queues must be in 1..32")
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)  # read when HIP initialises, below
     from rbmd_amd.lib import Context, GpuSystem
 
     if args.precision is None:

@@ -161,6 +161,7 @@ typedef struct rbmd_thermo {
 
 typedef struct rbmd_ctx rbmd_ctx;
 typedef struct rbmd_sys rbmd_sys;
+typedef struct rbmd_pack rbmd_pack;
 
 /* ---- context (replaces the VTK-m device tag switch, Application.cpp:38-130) ---- */
 rbmd_status rbmd_ctx_create(int32_t device, rbmd_ctx** out);
@@ -241,6 +242,26 @@ rbmd_status rbmd_md_wait(rbmd_sys* sys);
 rbmd_status rbmd_md_query(rbmd_sys* sys, int32_t* idle);
 rbmd_status rbmd_md_thermo(rbmd_sys* sys, rbmd_thermo* out);
 rbmd_status rbmd_md_energy(rbmd_sys* sys); /* evaluate e_long now (ComputeEwaldEnergy) */
+
+/* ---- packs: many systems, one launch per kernel of the step.  No counterpart
+ * in the reference (its executioner, Executioner.cpp:62-129, runs one system);
+ * each system's step is the ExecutionNVT::Solve sequence of rbmd_md_step
+ * (ExecutionNVT.cpp:90-135) with the same bits as stepping it alone.
+ * Systems of a pack share the context, have run rbmd_md_init, use the VERLET
+ * list path (pair_path LISTS) with at most 16 types, NOSE_HOOVER or NONE
+ * thermostats, energy_every 0, sampled (not injected) RBE k vectors or Ewald,
+ * and agree on force field, special-bond mode, precision, SHAKE and
+ * thermostat; sizes, boxes and step counts may differ.  A pack keeps the
+ * systems' device pointers: do not resize, re-initialise or destroy a system
+ * while a pack holds it. ---- */
+rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n, rbmd_pack** out);
+rbmd_status rbmd_pack_destroy(rbmd_pack* pack);
+rbmd_status rbmd_pack_step(rbmd_pack* pack, int64_t n_steps);    /* n packed steps, launched kernel by kernel */
+rbmd_status rbmd_pack_enqueue(rbmd_pack* pack, int64_t n_steps); /* n replays of one captured packed step */
+/* wait for the pack's work; status_out[k] (optional) = status word of system k
+ * (bit0 non-finite, bit1 SHAKE, bit2 capacity); returns the first failing
+ * system's error */
+rbmd_status rbmd_pack_wait(rbmd_pack* pack, int32_t* status_out);
 
 /* ---- per-kernel timing with HIP events on the context stream ---- */
 rbmd_status rbmd_profile_enable(rbmd_ctx* ctx, int32_t on);

@@ -480,43 +480,71 @@ void compute_force(rbmd_sys* s, const double* kvec, int thermo) {
   }
 }
 
-// k vectors of one MD step (stream id = step index), staged through a
-// double-buffered pinned ring so host sampling overlaps device work
-rbmd_status kvec_for_step(rbmd_sys* s, int64_t step, const double** out) {
-  if (!has_k(s) || s->p.coulomb_type != RBMD_COULOMB_RBE) {
-    *out = s->kcur;
-    return RBMD_OK;
-  }
-  if (s->injected) {
-    *out = s->kcur;
-    return RBMD_OK;
-  }
-  const int R = s->ring_R;
-  const int64_t batch = step / R;
-  const int half = (int)(batch % 2);
-  const size_t per = (size_t)3 * s->P;
-  if (s->ring_have[half] != batch) {
-    HIP_TRY(hipEventSynchronize(s->stage_ev[half]));
-    double* buf = s->stage[half];
-    const int64_t first = batch * R;
-    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    std::vector<double> mtmp(per * R);
-    for (unsigned t = 0; t < nth; ++t) {
-      th.emplace_back([&, t] {
-        for (int r = (int)t; r < R; r += (int)nth) {
-          s->sampler.draw(s->p.seed, (uint64_t)(first + r), mtmp.data() + r * per);
-          s->sampler.to_k(mtmp.data() + r * per, buf + r * per);
-        }
-      });
+// Refill the k-vector rings of the systems whose ring half for `step` holds
+// another batch: the host draws the R steps' k vectors of every such system
+// (one thread pool over all (system, step) pairs) into the pinned staging
+// half, then each system's ring half is copied on its stream.  The staging
+// half is reused only after its previous copy completed (its event).
+// steps[k]: the step system k runs next.
+rbmd_status ring_fill(rbmd_sys* const* sv, const int64_t* steps, int nsys) {
+  std::vector<rbmd_sys*> todo;
+  std::vector<int64_t> tstep;
+  for (int k = 0; k < nsys; ++k) {
+    rbmd_sys* s = sv[k];
+    if (!has_k(s) || s->p.coulomb_type != RBMD_COULOMB_RBE || s->injected) continue;
+    const int64_t batch = steps[k] / s->ring_R;
+    if (s->ring_have[batch % 2] != batch) {
+      todo.push_back(s);
+      tstep.push_back(steps[k]);
     }
-    for (auto& t : th) t.join();
-    HIP_TRY(hipMemcpyAsync(s->kring + (size_t)half * R * per, buf, per * R * sizeof(double), hipMemcpyHostToDevice,
-                           s->ctx->stream));
+  }
+  if (todo.empty()) return RBMD_OK;
+  for (size_t k = 0; k < todo.size(); ++k)
+    HIP_TRY(hipEventSynchronize(todo[k]->stage_ev[(tstep[k] / todo[k]->ring_R) % 2]));
+  std::vector<std::pair<int, int>> work;  // (system, step within the batch)
+  for (int k = 0; k < (int)todo.size(); ++k)
+    for (int r = 0; r < todo[k]->ring_R; ++r) work.emplace_back(k, r);
+  const unsigned nth = std::max(1u, std::min<unsigned>(16u, std::min<unsigned>(std::thread::hardware_concurrency(),
+                                                                                  (unsigned)work.size())));
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nth; ++t) {
+    th.emplace_back([&, t] {
+      std::vector<double> m;
+      for (size_t w = t; w < work.size(); w += nth) {
+        rbmd_sys* s = todo[work[w].first];
+        const int r = work[w].second;
+        const int64_t batch = tstep[work[w].first] / s->ring_R;
+        const size_t per = (size_t)3 * s->P;
+        m.resize(per);
+        s->sampler.draw(s->p.seed, (uint64_t)(batch * s->ring_R + r), m.data());
+        s->sampler.to_k(m.data(), s->stage[batch % 2] + r * per);
+      }
+    });
+  }
+  for (auto& t : th) t.join();
+  for (size_t k = 0; k < todo.size(); ++k) {
+    rbmd_sys* s = todo[k];
+    const int64_t batch = tstep[k] / s->ring_R;
+    const int half = (int)(batch % 2);
+    const size_t per = (size_t)3 * s->P;
+    HIP_TRY(hipMemcpyAsync(s->kring + (size_t)half * s->ring_R * per, s->stage[half], per * s->ring_R * sizeof(double),
+                           hipMemcpyHostToDevice, s->ctx->stream));
     HIP_TRY(hipEventRecord(s->stage_ev[half], s->ctx->stream));
     s->ring_have[half] = batch;
   }
-  *out = s->kring + ((size_t)half * R + (size_t)(step % R)) * per;
+  return RBMD_OK;
+}
+
+// k vectors of one MD step (stream id = step index), staged through a
+// double-buffered pinned ring so host sampling overlaps device work
+rbmd_status kvec_for_step(rbmd_sys* s, int64_t step, const double** out) {
+  if (!has_k(s) || s->p.coulomb_type != RBMD_COULOMB_RBE || s->injected) {
+    *out = s->kcur;
+    return RBMD_OK;
+  }
+  RC_TRY(ring_fill(&s, &step, 1));
+  const int R = s->ring_R;
+  *out = s->kring + ((size_t)((step / R) % 2) * R + (size_t)(step % R)) * ((size_t)3 * s->P);
   return RBMD_OK;
 }
 
@@ -1640,6 +1668,283 @@ rbmd_status rbmd_profile_get(rbmd_ctx* c, int32_t i, const char** name, int64_t*
   if (launches) *launches = c->recs[i].launches;
   if (ms) *ms = c->recs[i].ms;
   return RBMD_OK;
+}
+
+// =================================================================== packs
+// Many systems of the same shape stepped together: every kernel of the MD
+// step is one launch over the blocks of all systems (PackLaunch), so small
+// systems no longer pay a full chain of launches each.  Each system keeps its
+// own buffers, thermostat state, k-vector ring and status word; its blocks
+// run exactly the grid of its own launches, so its results are the same bits
+// as rbmd_md_step / rbmd_md_run_graph on that system alone.
+
+struct rbmd_pack {
+  rbmd_ctx* ctx = nullptr;
+  std::vector<rbmd_sys*> sys;
+  PackSig sig{};
+  int tc = 0;
+  bool ring = false, nh = false, shake = false, scale_v = false, has_k = false, has_b = false;
+  PackSys* dsys = nullptr;
+  int* dmap = nullptr;
+  int* dfirst = nullptr;
+  int* dstatus = nullptr;
+  int* hstatus = nullptr;  // pinned
+  PackLaunch L[PK_COUNT];
+  int nblk[PK_COUNT] = {};
+  size_t lds_first = 0, lds_scatter = 0;
+  hipGraphExec_t graph = nullptr;
+  std::vector<void*> allocs;
+};
+
+static const char* const kPackName[PK_COUNT] = {
+    "pk_nhc",   "pk_kvec_slot", "pk_first_half", "pk_scan", "pk_scatter",     "pk_cell_sort",
+    "pk_one",   "pk_shake",     "pk_gather",     "pk_nbr_build", "pk_pair",   "pk_rho_partial",
+    "pk_rho_reduce", "pk_kforce", "pk_bonded",   "pk_second_half", "pk_scale_v", "pk_rattle"};
+
+// the thermo-row descriptor a system's own step reduces (rows per producer =
+// the grid of that producer's launch)
+static RowsDesc pack_rows_desc(rbmd_sys* s, bool has_kk, bool has_bb, bool shk) {
+  int used[NSEG];
+  used[SEG_PAIR] = pair_blocks(s->v);
+  used[SEG_K] = has_kk ? kforce_blocks(s->v) : 0;
+  used[SEG_BOND] = has_bb ? (s->n + kBlock - 1) / kBlock : 0;
+  used[SEG_KE] = (s->n + kBlock - 1) / kBlock;
+  used[SEG_SHAKE] = shk ? (s->nmol + kBlock - 1) / kBlock : 0;
+  RowsDesc d;
+  d.nseg = NSEG;
+  for (int g = 0; g < NSEG; ++g) d.seg[g] = RowSeg{s->seg_rows(g), used[g], kSegWidth[g], kSegSlot0[g], kSegGroups[g]};
+  return d;
+}
+
+static void pack_enqueue_step(rbmd_pack* p) {
+  rbmd_ctx* c = p->ctx;
+  hipStream_t st = c->stream;
+  auto integ = [&](int k, int arg, size_t lds) {
+    c->run(kPackName[k], [&] { launch_pack_integrate(k, p->L[k], p->nblk[k], p->sig, arg, lds, nullptr, st); });
+  };
+  auto cells = [&](int k, size_t lds) {
+    c->run(kPackName[k], [&] { launch_pack_cells(k, p->L[k], p->nblk[k], lds, st); });
+  };
+  auto ksp = [&](int k) { c->run(kPackName[k], [&] { launch_pack_kspace(k, p->L[k], p->nblk[k], p->sig, st); }); };
+  integ(PK_NHC, p->nh ? 1 : 5, 0);
+  if (p->ring) ksp(PK_KSLOT);
+  integ(PK_FIRST_HALF, p->tc == RBMD_TEMP_NOSE_HOOVER ? 1 : 0, p->lds_first);
+  cells(PK_SCAN, 0);
+  cells(PK_SCATTER, p->lds_scatter);
+  cells(PK_CELL_SORT, 0);
+  if (p->shake) {
+    integ(PK_ONE, 0, 0);
+    integ(PK_SHAKE, 0, 0);
+  }
+  cells(PK_GATHER, 0);
+  cells(PK_NBR, 0);
+  c->run(kPackName[PK_PAIR], [&] { launch_pack_pair(p->L[PK_PAIR], p->nblk[PK_PAIR], p->sig, st); });
+  if (p->has_k) {
+    ksp(PK_RHO);
+    ksp(PK_RHO_RED);
+    ksp(PK_KFORCE);
+  }
+  if (p->has_b) integ(PK_BONDED, 0, 0);
+  integ(PK_SECOND, 1, 0);
+  integ(PK_NHC, 2 | 8, 0);
+  if (p->scale_v) integ(PK_ATOMS, 0, 0);
+  if (p->shake) integ(PK_RATTLE, 0, 0);
+}
+
+static rbmd_status pack_fail_free(rbmd_pack* p, rbmd_status rc) {
+  for (void* q : p->allocs) (void)hipFree(q);
+  if (p->hstatus) (void)hipHostFree(p->hstatus);
+  delete p;
+  return rc;
+}
+
+rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n, rbmd_pack** out) {
+  if (!ctx || !systems || n <= 0 || !out) return fail(RBMD_ERR_ARG, "bad arguments");
+  rbmd_sys* s0 = systems[0];
+  if (!s0) return fail(RBMD_ERR_ARG, "null system 0");
+  auto mode_of = [](const rbmd_sys* s) { return s->v.lj_only ? 2 : (s->v.weighted ? 1 : 0); };
+  auto sp_of = [](const rbmd_sys* s) { return s->v.weighted && !s->v.lj_only ? (s->v.sp_max <= 4 ? 4 : 16) : 0; };
+  for (int k = 0; k < n; ++k) {
+    const rbmd_sys* s = systems[k];
+    if (!s) return fail(RBMD_ERR_ARG, "null system %d", k);
+    if (s->ctx != ctx) return fail(RBMD_ERR_ARG, "system %d belongs to another context", k);
+    if (!s->initialized) return fail(RBMD_ERR_STATE, "system %d: rbmd_md_init has not run", k);
+    if (s->v.rbl || s->fused) return fail(RBMD_ERR_UNSUPPORTED, "system %d: packs run the VERLET list path", k);
+    if (s->ntypes > kMaxTypes) return fail(RBMD_ERR_UNSUPPORTED, "system %d: more than %d types", k, kMaxTypes);
+    const int tc = s->p.temp_ctrl;
+    if (tc != RBMD_TEMP_NOSE_HOOVER && tc != RBMD_TEMP_NONE)
+      return fail(RBMD_ERR_UNSUPPORTED, "system %d: packs run Nose-Hoover or NVE steps", k);
+    if (s->p.energy_every > 0) return fail(RBMD_ERR_UNSUPPORTED, "system %d: energy_every must be 0 in a pack", k);
+    if (shake_on(s) && s->nfree > 0) return fail(RBMD_ERR_UNSUPPORTED, "system %d: atoms outside SHAKE triplets", k);
+    if (s->injected) return fail(RBMD_ERR_UNSUPPORTED, "system %d: injected k vectors", k);
+    if (mode_of(s) != mode_of(s0) || sp_of(s) != sp_of(s0) || s->v.mixed != s0->v.mixed || has_k(s) != has_k(s0) ||
+        s->p.coulomb_type != s0->p.coulomb_type || has_bonded(s) != has_bonded(s0) || shake_on(s) != shake_on(s0) ||
+        tc != s0->p.temp_ctrl)
+      return fail(RBMD_ERR_ARG, "system %d differs from system 0 in force field, precision, constraints or thermostat", k);
+    for (int j = 0; j < k; ++j)
+      if (systems[j] == s) return fail(RBMD_ERR_ARG, "system %d appears twice", k);
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  rbmd_pack* p = new rbmd_pack();
+  p->ctx = ctx;
+  p->sys.assign(systems, systems + n);
+  p->tc = s0->p.temp_ctrl;
+  p->has_k = has_k(s0);
+  p->has_b = has_bonded(s0);
+  p->shake = shake_on(s0);
+  p->ring = p->has_k && s0->p.coulomb_type == RBMD_COULOMB_RBE;
+  p->nh = true;
+  p->scale_v = p->tc == RBMD_TEMP_NOSE_HOOVER && !p->shake;
+  p->sig.pair_mode = mode_of(s0);
+  p->sig.pair_sp = sp_of(s0);
+  p->sig.f32 = s0->v.mixed;
+  p->sig.has_k = p->has_k;
+  p->sig.ewald = s0->p.coulomb_type == RBMD_COULOMB_EWALD;
+  p->sig.has_b = p->has_b;
+  auto dalloc = [&](void** q, size_t bytes) -> rbmd_status {
+    hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 4));
+    if (e != hipSuccess) return fail(RBMD_ERR_OOM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    p->allocs.push_back(*q);
+    return RBMD_OK;
+  };
+  std::vector<PackSys> tab(n);
+  std::vector<int> first((size_t)PK_COUNT * (n + 1));
+  std::vector<std::vector<int>> blocks(PK_COUNT, std::vector<int>(n, 0));
+  for (int k = 0; k < n; ++k) {
+    rbmd_sys* s = systems[k];
+    if (p->ring && !s->kgraph) {
+      rbmd_status rc = s->dalloc(&s->kgraph, 3 * (size_t)s->P);
+      if (rc != RBMD_OK) return pack_fail_free(p, rc);
+    }
+    PackSys& e = tab[k];
+    std::memset(&e, 0, sizeof(e));
+    e.v = s->v;
+    e.v.nslab = 1;
+    e.iv = s->iv;
+    e.kv = s->kv;
+    e.kv.P = s->P;
+    e.kv.kvec = p->ring ? s->kgraph : s->kcur;
+    e.kv.S_over_P = s->p.coulomb_type == RBMD_COULOMB_RBE ? s->S / (double)s->P : 1.0;
+    e.kv.f32 = s->v.mixed;
+    e.nhc = s->nhc;
+    e.np = s->np;
+    e.rd = pack_rows_desc(s, p->has_k, p->has_b, p->shake);
+    e.rows_pair = s->seg_rows(SEG_PAIR);
+    e.rows_k = s->seg_rows(SEG_K);
+    e.rows_bond = s->seg_rows(SEG_BOND);
+    e.rows_ke = s->seg_rows(SEG_KE);
+    e.rows_shake = s->seg_rows(SEG_SHAKE);
+    e.shake_iter = s->shake_iter;
+    e.kring = p->ring ? s->kring : nullptr;
+    e.ring_R = s->ring_R;
+    for (int q = 0; q < PK_COUNT; ++q) {
+      bool on = true;
+      if (q == PK_KSLOT) on = p->ring;
+      if (q == PK_RHO || q == PK_RHO_RED || q == PK_KFORCE) on = p->has_k;
+      if (q == PK_BONDED) on = p->has_b;
+      if (q == PK_SHAKE || q == PK_RATTLE) on = p->shake;
+      blocks[q][k] = on ? pack_blocks(s->v, s->iv, s->P, q) : 0;
+    }
+    if (s->v.lds_bins) {
+      p->lds_first = std::max(p->lds_first, (size_t)s->v.ncell * sizeof(int));
+      p->lds_scatter = std::max(p->lds_scatter, 2 * (size_t)s->v.ncell * sizeof(int));
+    }
+  }
+  // block maps: kernel q's map starts at moff[q]
+  std::vector<size_t> moff(PK_COUNT + 1, 0);
+  for (int q = 0; q < PK_COUNT; ++q) {
+    int64_t tot = 0;
+    for (int k = 0; k < n; ++k) {
+      first[(size_t)q * (n + 1) + k] = (int)tot;
+      tot += blocks[q][k];
+    }
+    if (tot > (int64_t)0x7fffffff) return pack_fail_free(p, fail(RBMD_ERR_UNSUPPORTED, "pack grid too large"));
+    first[(size_t)q * (n + 1) + n] = (int)tot;
+    p->nblk[q] = (int)tot;
+    moff[q + 1] = moff[q] + (size_t)tot;
+  }
+  std::vector<int> map(std::max<size_t>(moff[PK_COUNT], 1));
+  for (int q = 0; q < PK_COUNT; ++q)
+    for (int k = 0; k < n; ++k)
+      for (int b = 0; b < blocks[q][k]; ++b) map[moff[q] + first[(size_t)q * (n + 1) + k] + b] = k;
+  rbmd_status rc = RBMD_OK;
+  if (rc == RBMD_OK) rc = dalloc((void**)&p->dsys, sizeof(PackSys) * n);
+  if (rc == RBMD_OK) rc = dalloc((void**)&p->dmap, sizeof(int) * map.size());
+  if (rc == RBMD_OK) rc = dalloc((void**)&p->dfirst, sizeof(int) * first.size());
+  if (rc == RBMD_OK) rc = dalloc((void**)&p->dstatus, sizeof(int) * n);
+  if (rc != RBMD_OK) return pack_fail_free(p, rc);
+  if (hipHostMalloc((void**)&p->hstatus, sizeof(int) * n) != hipSuccess)
+    return pack_fail_free(p, fail(RBMD_ERR_OOM, "pinned status array"));
+  hipStream_t st = ctx->stream;
+  if (hipMemcpyAsync(p->dsys, tab.data(), sizeof(PackSys) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(p->dmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(p->dfirst, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return pack_fail_free(p, fail(RBMD_ERR_HIP, "pack table upload"));
+  for (int q = 0; q < PK_COUNT; ++q) p->L[q] = PackLaunch{p->dsys, p->dmap + moff[q], p->dfirst + (size_t)q * (n + 1)};
+  *out = p;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_pack_destroy(rbmd_pack* p) {
+  if (!p) return RBMD_OK;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  if (p->graph) (void)hipGraphExecDestroy(p->graph);
+  return pack_fail_free(p, RBMD_OK);
+}
+
+// n packed steps; graph: replay one captured packed step
+static rbmd_status pack_steps(rbmd_pack* p, int64_t n_steps, bool graph) {
+  if (!p) return fail(RBMD_ERR_ARG, "null pack");
+  hipStream_t st = p->ctx->stream;
+  if (graph && !p->graph) {
+    const bool prof = p->ctx->prof;
+    p->ctx->prof = false;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    pack_enqueue_step(p);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    p->ctx->prof = prof;
+    if (e != hipSuccess) return fail(RBMD_ERR_HIP, "pack step capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(RBMD_ERR_HIP, "pack graph instantiate: %s", hipGetErrorString(e));
+  }
+  std::vector<int64_t> next(p->sys.size());
+  for (int64_t k = 0; k < n_steps; ++k) {
+    for (size_t q = 0; q < p->sys.size(); ++q) next[q] = p->sys[q]->step + 1;
+    RC_TRY(ring_fill(p->sys.data(), next.data(), (int)p->sys.size()));
+    if (graph)
+      HIP_TRY(hipGraphLaunch(p->graph, st));
+    else
+      pack_enqueue_step(p);
+    for (rbmd_sys* s : p->sys) s->step += 1;
+  }
+  return check_launch("pack steps");
+}
+
+rbmd_status rbmd_pack_step(rbmd_pack* p, int64_t n_steps) { return pack_steps(p, n_steps, false); }
+
+rbmd_status rbmd_pack_enqueue(rbmd_pack* p, int64_t n_steps) { return pack_steps(p, n_steps, true); }
+
+rbmd_status rbmd_pack_wait(rbmd_pack* p, int32_t* status_out) {
+  if (!p) return fail(RBMD_ERR_ARG, "null pack");
+  hipStream_t st = p->ctx->stream;
+  launch_pack_integrate(PK_ONE, p->L[PK_ONE], p->nblk[PK_ONE], p->sig, 1, 0, p->dstatus, st);
+  RC_TRY(check_launch("pack status"));
+  const int n = (int)p->sys.size();
+  HIP_TRY(hipMemcpyAsync(p->hstatus, p->dstatus, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int first_bad = -1;
+  for (int k = 0; k < n; ++k) {
+    if (status_out) status_out[k] = p->hstatus[k];
+    if (p->hstatus[k] && first_bad < 0) first_bad = k;
+  }
+  if (first_bad < 0) return RBMD_OK;
+  rbmd_status rc = status_to_rc(p->hstatus[first_bad]);
+  std::string msg = g_last_error;
+  return fail(rc, "pack system %d: %s", first_bad, msg.c_str());
 }
 
 // sizes of the ABI structs, checked from Python against the ctypes mirror

@@ -214,4 +214,74 @@ struct NhcParams {
 // (FinalIntegrate); 3: SetUp; 4: Berendsen/rescale end of step; 5: step++ only
 void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st);
 
+// ---------------------------------------------------------------- packs
+// A pack runs the MD steps of many independent systems with one launch per
+// kernel of the step: the blocks of every system sit back to back in one
+// grid, and each block finds its system and its block index within that
+// system's own grid, so every system runs exactly the decomposition (and the
+// arithmetic) of its own launches.
+
+// One system's arguments: what its own launches pass.
+struct PackSys {
+  SysView v;
+  IntView iv;
+  KView kv;               // kv.kvec: this step's k vectors
+  NhcState* nhc;
+  NhcParams np;
+  RowsDesc rd;            // thermo rows, reduced by the NHC phase 2 launch
+  double *rows_pair, *rows_k, *rows_bond, *rows_ke, *rows_shake;
+  int* shake_iter;
+  const double* kring;    // RBE k-vector ring [2][R][3P] (null: fixed k vectors)
+  int ring_R;
+};
+
+// One packed launch: map[b] = system of block b, first[k] = first block of
+// system k (first[nsys] = total)
+struct PackLaunch {
+  const PackSys* sys;
+  const int* map;
+  const int* first;
+};
+
+// kernels of the packed step, each with its own block map
+enum PackKernel {
+  PK_NHC,
+  PK_KSLOT,
+  PK_FIRST_HALF,
+  PK_SCAN,
+  PK_SCATTER,
+  PK_CELL_SORT,
+  PK_ONE,  // one small workgroup per system (counter resets, status gather)
+  PK_SHAKE,
+  PK_GATHER,
+  PK_NBR,
+  PK_PAIR,
+  PK_RHO,
+  PK_RHO_RED,
+  PK_KFORCE,
+  PK_BONDED,
+  PK_SECOND,
+  PK_ATOMS,  // one thread per atom (velocity scale)
+  PK_RATTLE,
+  PK_COUNT
+};
+
+// blocks of system s in each packed kernel (the grids of its own launches)
+int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel);
+
+struct PackSig {  // launch-wide choices, equal for every system of a pack
+  int pair_mode, pair_sp, f32, has_k, ewald, has_b;
+};
+
+// Packed launches of one kernel (nblocks = total over the pack; lds = the
+// largest dynamic LDS any system's own launch uses).  arg: PK_NHC the phase
+// (+8: reduce the thermo rows), PK_FIRST_HALF 1 = apply the NHC scale,
+// PK_SECOND the kick flag, PK_ONE 0 = zero the SHAKE iteration counters,
+// 1 = copy every status word to out[k].
+void launch_pack_cells(int kernel, const PackLaunch& L, int nblocks, size_t lds, hipStream_t st);
+void launch_pack_pair(const PackLaunch& L, int nblocks, const PackSig& g, hipStream_t st);
+void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const PackSig& g, hipStream_t st);
+void launch_pack_integrate(int kernel, const PackLaunch& L, int nblocks, const PackSig& g, int arg, size_t lds,
+                           int* out, hipStream_t st);
+
 }  // namespace rbmd

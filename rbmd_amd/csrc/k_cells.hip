@@ -16,6 +16,7 @@
 // broadcast to all lanes.  Lists are column-major [slot][n] so that the pair
 // kernel's per-lane reads of slot k are coalesced.
 #include "dev.hpp"
+#include "pack.hpp"
 #include "reduce.hpp"
 #include "cells.hpp"
 
@@ -93,7 +94,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds_wave, int* t
 
 // one workgroup: cell_start = exclusive scan of counts; cursor = cell_start;
 // work items (cell, 64-atom chunk); counts reset to zero for the next binning
-__global__ __launch_bounds__(1024) void k_scan(SysView s) {
+__device__ __forceinline__ void scan_body(const SysView& s, const int bid) {
   __shared__ int lds[17];
   int carry = 0, icarry = 0;
   for (int base = 0; base < s.ncell; base += blockDim.x) {
@@ -124,6 +125,10 @@ __global__ __launch_bounds__(1024) void k_scan(SysView s) {
   }
 }
 
+__global__ __launch_bounds__(1024) void k_scan(SysView s) {
+  scan_body(s, (int)blockIdx.x);
+}
+
 void launch_scan(const SysView& s, hipStream_t st) {
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s);
 }
@@ -131,9 +136,9 @@ void launch_scan(const SysView& s, hipStream_t st) {
 // Scatter atom ids into their cells.  With LDS aggregation each workgroup
 // reserves one range per cell with a single global atomic and places its atoms
 // by their LDS rank; the order inside a cell is fixed afterwards by k_cell_sort.
-__global__ __launch_bounds__(kBlock) void k_scatter(SysView s) {
+__device__ __forceinline__ void scatter_body(const SysView& s, const int bid) {
   extern __shared__ int lds[];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = bid * blockDim.x + threadIdx.x;
   int cell = i < s.n ? s.cell_of[i] : -1;
   if (!s.lds_bins) {
     if (cell >= 0) s.point_ids[atomicAdd(&s.cursor[cell], 1)] = i;
@@ -153,6 +158,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SysView s) {
   if (cell >= 0) s.point_ids[base[cell] + local] = i;
 }
 
+__global__ __launch_bounds__(kBlock) void k_scatter(SysView s) {
+  scatter_body(s, (int)blockIdx.x);
+}
+
 void launch_scatter(const SysView& s, hipStream_t st) {
   size_t lds = s.lds_bins ? 2 * (size_t)s.ncell * sizeof(int) : 0;
   hipLaunchKernelGGL(k_scatter, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), lds, st, s);
@@ -161,9 +170,9 @@ void launch_scatter(const SysView& s, hipStream_t st) {
 constexpr int kSortCap = 4096;
 
 // one workgroup per cell: rank each id by the number of smaller ids in the cell
-__global__ __launch_bounds__(kBlock) void k_cell_sort(SysView s) {
+__device__ __forceinline__ void cell_sort_body(const SysView& s, const int bid) {
   __shared__ int ids[kSortCap];
-  int c = blockIdx.x;
+  int c = bid;
   int start = s.cell_start[c];
   int cnt = s.cell_start[c + 1] - start;
   if (cnt > kSortCap) {
@@ -181,6 +190,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(SysView s) {
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_cell_sort(SysView s) {
+  cell_sort_body(s, (int)blockIdx.x);
+}
+
 void launch_cell_sort(const SysView& s, hipStream_t st) {
   hipLaunchKernelGGL(k_cell_sort, dim3(s.ncell), dim3(kBlock), 0, st, s);
 }
@@ -188,8 +201,8 @@ void launch_cell_sort(const SysView& s, hipStream_t st) {
 // sorted copies of positions / charges / types; misfit = current position is in
 // a different cell than the binned one (possible after SHAKE moves atoms, or at
 // x == hi where binning clamps and InWhichCell does not)
-__global__ __launch_bounds__(kBlock) void k_gather(SysView s) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void gather_body(const SysView& s, const int bid) {
+  int p = bid * blockDim.x + threadIdx.x;
   if (p >= s.n) return;
   int id = s.point_ids[p];
   double px = s.x[id], py = s.y[id], pz = s.z[id];
@@ -205,6 +218,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(SysView s) {
   const int mf = (ua != a || ub != b || uc != c) ? 1 : 0;
   s.misfit[p] = mf;
   if (mf) s.misfit_list[atomicAdd(s.misfit_count, 1)] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(SysView s) {
+  gather_body(s, (int)blockIdx.x);
 }
 
 void launch_gather(const SysView& s, hipStream_t st) {
@@ -232,11 +249,11 @@ void launch_gather(const SysView& s, hipStream_t st) {
 // accepted ones are written at their ballot prefix position (stream order).
 constexpr int kColWaves = 9;
 
-__device__ void misfit_block(const SysView& s, int wave, int lane) {
+__device__ void misfit_block(const SysView& s, int wave, int lane, int bid) {
   const int count = s.misfit_count[0];
   const int a = wave / 3 - 1, b = wave % 3 - 1;
   const int seg0 = wave * s.seg_cap;
-  for (int m = blockIdx.x - s.n_items; m < count; m += kMisfitBlocks) {
+  for (int m = bid - s.n_items; m < count; m += kMisfitBlocks) {
     const int i = __builtin_amdgcn_readfirstlane(s.misfit_list[m]);
     const double4 pi = s.spos[i];
     const int ca = cell_axis(pi.x, s.lo[0], s.dxdydz[0]);
@@ -271,16 +288,16 @@ __device__ void misfit_block(const SysView& s, int wave, int lane) {
   }
 }
 
-__global__ __launch_bounds__(kColWaves * kWave) void k_nbr_build_verlet(SysView s) {
+__device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const int bid) {
   __shared__ double4 stage[kColWaves][kWave];
   __shared__ int stage_e[kColWaves][kWave];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
-  if ((int)blockIdx.x >= s.n_items) {
-    misfit_block(s, wave, lane);
+  if (bid >= s.n_items) {
+    misfit_block(s, wave, lane, bid);
     return;
   }
-  const int item_idx = blockIdx.x;
+  const int item_idx = bid;
   if (item_idx >= s.n_items_dev[0]) return;
   const int item = __builtin_amdgcn_readfirstlane(s.items[item_idx]);
   const int c = item >> 8;
@@ -367,6 +384,10 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_nbr_build_verlet(SysView 
   }
   if (cnt > seg_cap) atomicOr(s.status, 4);
   if (active) s.nbr_num[wave * s.n + i] = cnt < seg_cap ? cnt : seg_cap;
+}
+
+__global__ __launch_bounds__(kColWaves * kWave) void k_nbr_build_verlet(SysView s) {
+  nbr_build_verlet_body(s, (int)blockIdx.x);
 }
 
 // RBL list state: core slots [0, rs_num), kept shell slots from rs_num (every
@@ -525,6 +546,57 @@ void launch_nbr_build(const SysView& s, hipStream_t st) {
     hipLaunchKernelGGL(k_nbr_misfit_rbl, dim3((waves * kWave + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
   } else {
     hipLaunchKernelGGL(k_nbr_build_verlet, dim3(s.n_items + kMisfitBlocks), dim3(kColWaves * kWave), 0, st, s);
+  }
+}
+
+// ------------------------------------------------------------------ packs
+
+__global__ __launch_bounds__(1024) void kp_scan(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  scan_body(e.v, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_scatter(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  scatter_body(e.v, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_cell_sort(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  cell_sort_body(e.v, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_gather(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  gather_body(e.v, bid);
+}
+
+__global__ __launch_bounds__(kColWaves * kWave) void kp_nbr_build(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  nbr_build_verlet_body(e.v, bid);
+}
+
+void launch_pack_cells(int kernel, const PackLaunch& L, int nblocks, size_t lds, hipStream_t st) {
+  if (nblocks <= 0) return;
+  const dim3 grid(nblocks);
+  switch (kernel) {
+    case PK_SCAN:
+      hipLaunchKernelGGL(kp_scan, grid, dim3(1024), 0, st, PACK_ARGS(L));
+      break;
+    case PK_SCATTER:
+      hipLaunchKernelGGL(kp_scatter, grid, dim3(kBlock), lds, st, PACK_ARGS(L));
+      break;
+    case PK_CELL_SORT:
+      hipLaunchKernelGGL(kp_cell_sort, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    case PK_GATHER:
+      hipLaunchKernelGGL(kp_gather, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    case PK_NBR:
+      hipLaunchKernelGGL(kp_nbr_build, grid, dim3(kColWaves * kWave), 0, st, PACK_ARGS(L));
+      break;
+    default:
+      break;
   }
 }
 

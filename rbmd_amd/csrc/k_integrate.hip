@@ -10,14 +10,15 @@
 //   NHCTempIntegrate      ExecutionNVT.cpp:1172-1256 (run on the device by one thread)
 //   Compute_Pressure      ExecutionNVT.cpp:1258-1276, ComputeVirial ExecutionMD.cpp:1263-1366
 #include "dev.hpp"
+#include "pack.hpp"
 #include "reduce.hpp"
 
 namespace rbmd {
 
 // NHC scale (optional), kick, drift, PBC, binning
-__global__ __launch_bounds__(kBlock) void k_first_half(SysView s, IntView iv, const double* scale, int do_bin) {
+__device__ __forceinline__ void first_half_body(const SysView& s, const IntView& iv, const double* scale, int do_bin, const int bid) {
   extern __shared__ int lds_cells[];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = bid * blockDim.x + threadIdx.x;
   int cell = -1;
   if (i < s.n) {
     double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
@@ -69,6 +70,10 @@ __global__ __launch_bounds__(kBlock) void k_first_half(SysView s, IntView iv, co
   if (do_bin) count_cells(s, cell, lds_cells);
 }
 
+__global__ __launch_bounds__(kBlock) void k_first_half(SysView s, IntView iv, const double* scale, int do_bin) {
+  first_half_body(s, iv, scale, do_bin, (int)blockIdx.x);
+}
+
 void launch_first_half(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin) {
   size_t lds = (do_bin && s.lds_bins) ? (size_t)s.ncell * sizeof(int) : 0;
   hipLaunchKernelGGL(k_first_half, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), lds, st, s, iv, scale, do_bin);
@@ -97,8 +102,8 @@ __device__ __forceinline__ void st3(double* x, double* y, double* z, int i, D3 v
 }
 
 // SHAKE on (old x, old v, F) of one H-O-H triplet; writes x, v, image flags
-__global__ __launch_bounds__(kBlock) void k_shake(SysView s, IntView iv, double* rows, int* max_iter_out) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void shake_body(const SysView& s, const IntView& iv, double* rows, int* max_iter_out, const int bid) {
+  int a = bid * blockDim.x + threadIdx.x;
   double vir[6] = {0, 0, 0, 0, 0, 0};
   int my_iter = 0;
   if (a < iv.nmol) {
@@ -220,7 +225,11 @@ __global__ __launch_bounds__(kBlock) void k_shake(SysView s, IntView iv, double*
   }
   if (max_iter_out) atomicMax(max_iter_out, my_iter);
   const double vals[kRowShake] = {vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
-  block_write_row<kRowShake>(vals, rows + (size_t)blockIdx.x * kRowShake);
+  block_write_row<kRowShake>(vals, rows + (size_t)bid * kRowShake);
+}
+
+__global__ __launch_bounds__(kBlock) void k_shake(SysView s, IntView iv, double* rows, int* max_iter_out) {
+  shake_body(s, iv, rows, max_iter_out, (int)blockIdx.x);
 }
 
 // atoms outside every SHAKE triplet get x = x_old, v = v_old: the reference
@@ -252,8 +261,8 @@ void launch_unconstrained_copy(const SysView& s, hipStream_t st, const int* free
 }
 
 // RATTLE on (x, old v, F)
-__global__ __launch_bounds__(kBlock) void k_rattle(SysView s, IntView iv) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rattle_body(const SysView& s, const IntView& iv, const int bid) {
+  int a = bid * blockDim.x + threadIdx.x;
   if (a >= iv.nmol) return;
   const double dt = iv.dt, fmt2v = iv.fmt2v;
   int i0 = iv.angle_atoms[3 * a], i1 = iv.angle_atoms[3 * a + 1], i2 = iv.angle_atoms[3 * a + 2];
@@ -307,6 +316,10 @@ __global__ __launch_bounds__(kBlock) void k_rattle(SysView s, IntView iv) {
   st3(s.vx, s.vy, s.vz, i2, add(sv[2], vc2));
 }
 
+__global__ __launch_bounds__(kBlock) void k_rattle(SysView s, IntView iv) {
+  rattle_body(s, iv, (int)blockIdx.x);
+}
+
 void launch_rattle(const SysView& s, const IntView& iv, hipStream_t st) {
   int grid = (iv.nmol + kBlock - 1) / kBlock;
   if (grid > 0) hipLaunchKernelGGL(k_rattle, dim3(grid), dim3(kBlock), 0, st, s, iv);
@@ -315,8 +328,8 @@ void launch_rattle(const SysView& s, const IntView& iv, hipStream_t st) {
 // bond + angle forces gathered per atom over its incident terms; the energy
 // and virial of a term are counted by its first atom
 template <bool THERMO>
-__global__ __launch_bounds__(kBlock) void k_bonded(SysView s, IntView iv, double* rows) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bonded_body(const SysView& s, const IntView& iv, double* rows, const int bid) {
+  int i = bid * blockDim.x + threadIdx.x;
   double f[3] = {0, 0, 0};
   double eb = 0, ea = 0;
   double vb[6] = {0, 0, 0, 0, 0, 0}, va[6] = {0, 0, 0, 0, 0, 0};
@@ -387,8 +400,13 @@ __global__ __launch_bounds__(kBlock) void k_bonded(SysView s, IntView iv, double
     // slots S_VIR_BOND .. S_VIR_ANGLE + 5, S_EBOND, S_EANGLE
     const double vals[kRowBond] = {vb[0], vb[1], vb[2], vb[3], vb[4], vb[5], va[0],
                                    va[1], va[2], va[3], va[4], va[5], eb,    ea};
-    block_write_row<kRowBond>(vals, rows + (size_t)blockIdx.x * kRowBond);
+    block_write_row<kRowBond>(vals, rows + (size_t)bid * kRowBond);
   }
+}
+
+template <bool THERMO>
+__global__ __launch_bounds__(kBlock) void k_bonded(SysView s, IntView iv, double* rows) {
+  bonded_body<THERMO>(s, iv, rows, (int)blockIdx.x);
 }
 
 void launch_bonded(const SysView& s, const IntView& iv, hipStream_t st, int thermo, double* rows, int* nblocks) {
@@ -402,9 +420,8 @@ void launch_bonded(const SysView& s, const IntView& iv, hipStream_t st, int ther
 
 // F = F_real[rank] (slab partials in order) + F_k + F_bonded; optional second kick;
 // kinetic energy row
-__global__ __launch_bounds__(kBlock) void k_second_half(SysView s, IntView iv, int kick, int has_k, int has_b,
-                                                        double* rows) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void second_half_body(const SysView& s, const IntView& iv, int kick, int has_k, int has_b, double* rows, const int bid) {
+  int i = bid * blockDim.x + threadIdx.x;
   double ke = 0.0;
   if (i < s.n) {
     int r = s.rank[i];
@@ -445,7 +462,11 @@ __global__ __launch_bounds__(kBlock) void k_second_half(SysView s, IntView iv, i
     if (!isfinite(fx + fy + fz)) atomicOr(s.status, 1);
   }
   const double vals[kRowKe] = {ke};
-  block_write_row<kRowKe>(vals, rows + (size_t)blockIdx.x * kRowKe);
+  block_write_row<kRowKe>(vals, rows + (size_t)bid * kRowKe);
+}
+
+__global__ __launch_bounds__(kBlock) void k_second_half(SysView s, IntView iv, int kick, int has_k, int has_b, double* rows) {
+  second_half_body(s, iv, kick, has_k, has_b, rows, (int)blockIdx.x);
 }
 
 void launch_second_half(const SysView& s, const IntView& iv, hipStream_t st, int kick, int has_k, int has_b,
@@ -467,13 +488,17 @@ void launch_ke_rows(const SysView& s, const IntView& iv, hipStream_t st, double*
   hipLaunchKernelGGL(k_ke_rows, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, iv, rows);
 }
 
-__global__ __launch_bounds__(kBlock) void k_scale_v(SysView s, const double* scale) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void scale_v_body(const SysView& s, const double* scale, const int bid) {
+  int i = bid * blockDim.x + threadIdx.x;
   if (i >= s.n) return;
   double f = *scale;
   s.vx[i] = f * s.vx[i];
   s.vy[i] = f * s.vy[i];
   s.vz[i] = f * s.vz[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_v(SysView s, const double* scale) {
+  scale_v_body(s, scale, (int)blockIdx.x);
 }
 
 void launch_scale_v(const SysView& s, hipStream_t st, const double* scale) {
@@ -562,43 +587,10 @@ __device__ void compute_pressure(NhcState& S, const NhcParams& p, const double* 
   S.e_angle = p.na > 0 ? sums[S_EANGLE] / p.na : 0.0;
 }
 
-// One workgroup: reduce the partial rows (each wavefront owns a set of slots,
-// lanes stride the rows, fixed-order shuffle tree), then thread 0 advances the
-// thermostat state.
-__global__ __launch_bounds__(1024) void k_nhc(NhcState* state, NhcParams p, RowsDesc rd, int phase) {
-  // All producer segments at once: segment g owns threads [base, base +
-  // groups * width); thread (group, k) strides rows group, group + groups, ...
-  // of column k (consecutive threads read consecutive doubles), then thread k
-  // of the segment adds the group partials in order.  Fixed order throughout.
-  __shared__ double part[1024];
-  __shared__ double sums[S_NSLOT];
-  const int tid = threadIdx.x;
-  if (tid < S_NSLOT) sums[tid] = 0.0;
-  int base = 0, mine = -1, mbase = 0;
-  for (int g = 0; g < rd.nseg; ++g) {
-    const int span = rd.seg[g].groups * rd.seg[g].width;
-    if (tid >= base && tid < base + span) {
-      mine = g;
-      mbase = base;
-    }
-    base += span;
-  }
-  if (mine >= 0) {
-    const RowSeg sg = rd.seg[mine];
-    const int k = (tid - mbase) % sg.width, grp = (tid - mbase) / sg.width;
-    double acc = 0.0;
-    for (int r = grp; r < sg.nrows; r += sg.groups) acc += sg.rows[(size_t)r * sg.width + k];
-    part[tid] = acc;
-  }
-  __syncthreads();
-  if (mine >= 0 && tid - mbase < rd.seg[mine].width) {
-    const RowSeg sg = rd.seg[mine];
-    double acc = 0.0;
-    for (int grp = 0; grp < sg.groups; ++grp) acc += part[mbase + grp * sg.width + (tid - mbase)];
-    sums[sg.slot0 + (tid - mbase)] = acc;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
+// Thread 0's thermostat step from the reduced sums.  Not inlined: the
+// single-system and packed NHC kernels run this one instruction sequence, so
+// their chain variables agree to the bit.
+__device__ __noinline__ void nhc_update(NhcState* state, const NhcParams& p, const double* sums, int phase) {
   NhcState S = *state;
   switch (phase) {
     case 3:  // SetUp: ComputeTempe, ComputeTempTarget, chain masses (ExecutionNVT.cpp:1106-1122)
@@ -649,8 +641,138 @@ __global__ __launch_bounds__(1024) void k_nhc(NhcState* state, NhcParams p, Rows
   *state = S;
 }
 
+// One workgroup: reduce the partial rows (each wavefront owns a set of slots,
+// lanes stride the rows, fixed-order shuffle tree), then thread 0 advances the
+// thermostat state.
+// nseg: thermo row segments to reduce (rd.nseg, or 0 for no reduction)
+__device__ __forceinline__ void nhc_body(NhcState* state, const NhcParams& p, const RowsDesc& rd, const int nseg,
+                                         int phase, const int bid) {
+  // All producer segments at once: segment g owns threads [base, base +
+  // groups * width); thread (group, k) strides rows group, group + groups, ...
+  // of column k (consecutive threads read consecutive doubles), then thread k
+  // of the segment adds the group partials in order.  Fixed order throughout.
+  __shared__ double part[1024];
+  __shared__ double sums[S_NSLOT];
+  const int tid = threadIdx.x;
+  if (tid < S_NSLOT) sums[tid] = 0.0;
+  int base = 0, mine = -1, mbase = 0;
+  for (int g = 0; g < nseg; ++g) {
+    const int span = rd.seg[g].groups * rd.seg[g].width;
+    if (tid >= base && tid < base + span) {
+      mine = g;
+      mbase = base;
+    }
+    base += span;
+  }
+  if (mine >= 0) {
+    const RowSeg sg = rd.seg[mine];
+    const int k = (tid - mbase) % sg.width, grp = (tid - mbase) / sg.width;
+    double acc = 0.0;
+    for (int r = grp; r < sg.nrows; r += sg.groups) acc += sg.rows[(size_t)r * sg.width + k];
+    part[tid] = acc;
+  }
+  __syncthreads();
+  if (mine >= 0 && tid - mbase < rd.seg[mine].width) {
+    const RowSeg sg = rd.seg[mine];
+    double acc = 0.0;
+    for (int grp = 0; grp < sg.groups; ++grp) acc += part[mbase + grp * sg.width + (tid - mbase)];
+    sums[sg.slot0 + (tid - mbase)] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  nhc_update(state, p, sums, phase);
+}
+
+__global__ __launch_bounds__(1024) void k_nhc(NhcState* state, NhcParams p, RowsDesc rd, int phase) {
+  nhc_body(state, p, rd, rd.nseg, phase, (int)blockIdx.x);
+}
+
 void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st) {
   hipLaunchKernelGGL(k_nhc, dim3(1), dim3(1024), 0, st, st_dev, p, rd, phase);
+}
+
+// ------------------------------------------------------------------ packs
+// The same bodies as the kernels above, one launch over every system of a
+// pack; each block takes its system's arguments from the pack table.
+
+__global__ __launch_bounds__(1024) void kp_nhc(PACK_PARAMS, int phase, int reduce) {
+  PACK_ENTRY(e, bid)
+  nhc_body(e.nhc, e.np, e.rd, reduce ? e.rd.nseg : 0, phase, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_first_half(PACK_PARAMS, int use_scale) {
+  PACK_ENTRY(e, bid)
+  first_half_body(e.v, e.iv, use_scale ? &e.nhc->factor : nullptr, 1, bid);
+}
+
+// what 0: zero the SHAKE iteration counter; 1: copy the status word to out[k]
+__global__ __launch_bounds__(kWave) void kp_one(PACK_PARAMS, int what, int* out) {
+  PACK_ENTRY(e, bid)
+  if (threadIdx.x != 0) return;
+  if (what == 0) {
+    if (e.shake_iter) *e.shake_iter = 0;
+  } else {
+    out[PACK_INDEX()] = *e.v.status;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kp_shake(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  shake_body(e.v, e.iv, e.rows_shake, e.shake_iter, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_bonded(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  bonded_body<true>(e.v, e.iv, e.rows_bond, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_second_half(PACK_PARAMS, int kick, int has_k, int has_b) {
+  PACK_ENTRY(e, bid)
+  second_half_body(e.v, e.iv, kick, has_k, has_b, e.rows_ke, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_scale_v(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  scale_v_body(e.v, &e.nhc->factor, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_rattle(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  rattle_body(e.v, e.iv, bid);
+}
+
+void launch_pack_integrate(int kernel, const PackLaunch& L, int nblocks, const PackSig& g, int arg, size_t lds,
+                           int* out, hipStream_t st) {
+  if (nblocks <= 0) return;
+  const dim3 grid(nblocks);
+  switch (kernel) {
+    case PK_NHC:
+      hipLaunchKernelGGL(kp_nhc, grid, dim3(1024), 0, st, PACK_ARGS(L), arg & 7, (arg >> 3) & 1);
+      break;
+    case PK_FIRST_HALF:
+      hipLaunchKernelGGL(kp_first_half, grid, dim3(kBlock), lds, st, PACK_ARGS(L), arg);
+      break;
+    case PK_ONE:
+      hipLaunchKernelGGL(kp_one, grid, dim3(kWave), 0, st, PACK_ARGS(L), arg, out);
+      break;
+    case PK_SHAKE:
+      hipLaunchKernelGGL(kp_shake, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    case PK_BONDED:
+      hipLaunchKernelGGL(kp_bonded, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    case PK_SECOND:
+      hipLaunchKernelGGL(kp_second_half, grid, dim3(kBlock), 0, st, PACK_ARGS(L), arg, g.has_k, g.has_b);
+      break;
+    case PK_ATOMS:
+      hipLaunchKernelGGL(kp_scale_v, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    case PK_RATTLE:
+      hipLaunchKernelGGL(kp_rattle, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    default:
+      break;
+  }
 }
 
 }  // namespace rbmd

@@ -12,7 +12,9 @@
 //   (ExecutionNVT.cpp:718).  With an Ewald prefactor e^{-k^2/4 alpha} and scale 1
 //   the same kernel is ComputeNewFarElectrostaticsWorklet (RunWorklet.cpp:2492-2540).
 #include "dev.hpp"
+#include "pack.hpp"
 #include "reduce.hpp"
+#include "cells.hpp"
 #include "sincos.hpp"
 
 namespace rbmd {
@@ -44,10 +46,11 @@ __device__ __forceinline__ void phase_sincos(double mx, double my, double mz, do
 // reciprocal rounds to the same integer as the quotient
 __device__ __forceinline__ double k_to_m(double k, double box) { return rint(k * box * (0.5 / kPi)); }
 
+// bk: block of kRhoK k vectors, ba: block of kBlock * kRhoAtomsPerThread atoms
 template <bool F32>
-__global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
+__device__ __forceinline__ void rho_partial_body(const SysView& s, const KView& kv, const int bk, const int ba) {
   __shared__ double lds[kBlock / kWave][2 * kRhoK];
-  const int l0 = blockIdx.x * kRhoK;  // k-vector block on x (a converged Ewald grid has millions)
+  const int l0 = bk * kRhoK;
   double re[kRhoK], im[kRhoK];
 #pragma unroll
   for (int t = 0; t < kRhoK; ++t) re[t] = im[t] = 0.0;
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
     my[t] = ok ? k_to_m(kv.kvec[3 * l + 1], s.box[1]) : 0.0;
     mz[t] = ok ? k_to_m(kv.kvec[3 * l + 2], s.box[2]) : 0.0;
   }
-  const int base = blockIdx.y * kBlock * kRhoAtomsPerThread;
+  const int base = ba * kBlock * kRhoAtomsPerThread;
   const double ibx = 1.0 / s.box[0], iby = 1.0 / s.box[1], ibz = 1.0 / s.box[2];
 #pragma unroll
   for (int a = 0; a < kRhoAtomsPerThread; ++a) {
@@ -94,27 +97,41 @@ __global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
     if (l < kv.P) {
       double acc = 0.0;
       for (int w = 0; w < kBlock / kWave; ++w) acc += lds[w][threadIdx.x];
-      kv.rho_part[(size_t)blockIdx.y * 2 * kv.P + 2 * l + (threadIdx.x & 1)] = acc;
+      kv.rho_part[(size_t)ba * 2 * kv.P + 2 * l + (threadIdx.x & 1)] = acc;
     }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_rho_reduce(KView kv) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+// k-vector blocks on x (a converged Ewald grid has millions), atom blocks on y
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
+  rho_partial_body<F32>(s, kv, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+__device__ __forceinline__ void rho_reduce_body(const KView& kv, const int bid) {
+  int t = bid * blockDim.x + threadIdx.x;
   if (t >= 2 * kv.P) return;
   double acc = 0.0;
   for (int b = 0; b < kv.nblk_rho; ++b) acc += kv.rho_part[(size_t)b * 2 * kv.P + t];
   kv.rho[t] = acc;
 }
 
+__global__ __launch_bounds__(kBlock) void k_rho_reduce(KView kv) {
+  rho_reduce_body(kv, (int)blockIdx.x);
+}
+
 // the k vectors of the current step (device step counter) from the
 // double-buffered ring [2][R][3P] into a fixed buffer (step graphs read it)
-__global__ __launch_bounds__(kBlock) void k_kvec_slot(double* dst, const double* ring, int R, int per,
-                                                      const int64_t* step) {
+__device__ __forceinline__ void kvec_slot_body(double* dst, const double* ring, int R, int per, const int64_t* step,
+                                               const int bid, const int nblk) {
   const int64_t st = *step;
   const size_t slot = (size_t)((st / R) % 2) * R + (size_t)(st % R);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < per; t += gridDim.x * blockDim.x)
-    dst[t] = ring[slot * per + t];
+  for (int t = bid * blockDim.x + threadIdx.x; t < per; t += nblk * blockDim.x) dst[t] = ring[slot * per + t];
+}
+
+__global__ __launch_bounds__(kBlock) void k_kvec_slot(double* dst, const double* ring, int R, int per,
+                                                      const int64_t* step) {
+  kvec_slot_body(dst, ring, R, per, step, (int)blockIdx.x, (int)gridDim.x);
 }
 
 void launch_kvec_slot(double* dst, const double* ring, int R, int per, const int64_t* step, hipStream_t st) {
@@ -139,12 +156,12 @@ constexpr int kKLg = 2;  // 4 lanes per atom split the k list
 // prefactor pre_l = -4 pi / (V k^2) [* e^{-k^2/4alpha} for Ewald]; lane `sub`
 // of an atom takes k-vectors sub, sub + 4, ... of each LDS tile
 template <bool THERMO, bool PERATOM, bool F32>
-__global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewald, double alpha, double* rows) {
+__device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, int ewald, double alpha, double* rows, const int bid) {
   constexpr int L = 1 << kKLg;
   __shared__ double tk[kForceTile][4];
   __shared__ double tm[kForceTile][3];
   __shared__ double tr[kForceTile][2];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = bid * blockDim.x + threadIdx.x;
   const int i = t >> kKLg;
   const int sub = t & (L - 1);
   const bool act = i < s.n;
@@ -201,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewal
   const double sc = kv.S_over_P;
   if (THERMO) {
     const double vals[kRowK] = {v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc, v[4] * sc, v[5] * sc};
-    block_write_row<kRowK>(vals, rows + (size_t)blockIdx.x * kRowK);
+    block_write_row<kRowK>(vals, rows + (size_t)bid * kRowK);
   }
 #pragma unroll
   for (int off = 1; off < L; off <<= 1) {
@@ -218,6 +235,11 @@ __global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewal
     if (PERATOM)
       for (int k = 0; k < 6; ++k) s.vir_k[(size_t)6 * i + k] = v[k] * sc;
   }
+}
+
+template <bool THERMO, bool PERATOM, bool F32>
+__global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewald, double alpha, double* rows) {
+  kforce_body<THERMO, PERATOM, F32>(s, kv, ewald, alpha, rows, (int)blockIdx.x);
 }
 
 int kforce_blocks(const SysView& s) { return (int)((((size_t)s.n << kKLg) + kBlock - 1) / kBlock); }
@@ -241,6 +263,97 @@ void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, 
     hipLaunchKernelGGL((k_kforce<true, false, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
   else
     hipLaunchKernelGGL((k_kforce<false, false, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+}
+
+// ------------------------------------------------------------------ packs
+
+__global__ __launch_bounds__(kBlock) void kp_kvec_slot(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  kvec_slot_body(const_cast<double*>(e.kv.kvec), e.kring, e.ring_R, 3 * e.kv.P, &e.nhc->step, bid, PACK_GRID());
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void kp_rho_partial(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  const int kb = (e.kv.P + kRhoK - 1) / kRhoK;
+  rho_partial_body<F32>(e.v, e.kv, bid % kb, bid / kb);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_rho_reduce(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  rho_reduce_body(e.kv, bid);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void kp_kforce(PACK_PARAMS, int ewald) {
+  PACK_ENTRY(e, bid)
+  kforce_body<true, false, F32>(e.v, e.kv, ewald, e.v.alpha, e.rows_k, bid);
+}
+
+void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const PackSig& g, hipStream_t st) {
+  if (nblocks <= 0) return;
+  const dim3 grid(nblocks);
+  switch (kernel) {
+    case PK_KSLOT:
+      hipLaunchKernelGGL(kp_kvec_slot, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    case PK_RHO:
+      if (g.f32)
+        hipLaunchKernelGGL(kp_rho_partial<true>, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      else
+        hipLaunchKernelGGL(kp_rho_partial<false>, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    case PK_RHO_RED:
+      hipLaunchKernelGGL(kp_rho_reduce, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      break;
+    case PK_KFORCE:
+      if (g.f32)
+        hipLaunchKernelGGL(kp_kforce<true>, grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald);
+      else
+        hipLaunchKernelGGL(kp_kforce<false>, grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald);
+      break;
+    default:
+      break;
+  }
+}
+
+// The grid of each kernel of the step for one system, as its own launches
+// size it (k_first_half .. k_rattle above and in the other kernel files).
+int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel) {
+  const int atoms = (s.n + kBlock - 1) / kBlock;
+  const int mols = (iv.nmol + kBlock - 1) / kBlock;
+  switch (kernel) {
+    case PK_NHC:
+    case PK_SCAN:
+    case PK_ONE:
+      return 1;
+    case PK_KSLOT:
+      return (3 * P + kBlock - 1) / kBlock;
+    case PK_FIRST_HALF:
+    case PK_SCATTER:
+    case PK_GATHER:
+    case PK_BONDED:
+    case PK_SECOND:
+    case PK_ATOMS:
+      return atoms;
+    case PK_CELL_SORT:
+      return s.ncell;
+    case PK_SHAKE:
+    case PK_RATTLE:
+      return mols;
+    case PK_NBR:
+      return s.n_items + kMisfitBlocks;
+    case PK_PAIR:
+      return pair_blocks(s);
+    case PK_RHO:
+      return ((P + kRhoK - 1) / kRhoK) * ((s.n + kBlock * kRhoAtomsPerThread - 1) / (kBlock * kRhoAtomsPerThread));
+    case PK_RHO_RED:
+      return (2 * P + kBlock - 1) / kBlock;
+    case PK_KFORCE:
+      return kforce_blocks(s);
+    default:
+      return 0;
+  }
 }
 
 }  // namespace rbmd

@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "dev.hpp"
+#include "pack.hpp"
 #include "reduce.hpp"
 #include "pair.hpp"
 
@@ -66,7 +67,7 @@ __device__ __forceinline__ int cursor_take(ListCursor& c, const int (&cnt)[kMaxS
 // and the energies stay FP64 (each FP32 term is exact in FP64, so the sums
 // carry no more rounding than the FP64 path's).
 template <int MODE, bool THERMO, bool PERATOM, int SP, bool LTAB, bool F32>
-__global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows) {
+__device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows, const int bid) {
   constexpr bool EN = THERMO || PERATOM;
   constexpr int SPD = SP > 0 ? SP : 1;
   using TT = typename std::conditional<F32, float, double>::type;
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
   __syncthreads();
   const int lg = s.lanes_lg;
   const int L = 1 << lg;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = bid * blockDim.x + threadIdx.x;
   const int i = t >> lg;
   const int sub = t & (L - 1);
   double fx = 0, fy = 0, fz = 0;
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
   }
   if (THERMO) {
     const double vals[kRowPair] = {pelj, pec, vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
-    block_write_row<kRowPair>(vals, rows + (size_t)blockIdx.x * kRowPair);
+    block_write_row<kRowPair>(vals, rows + (size_t)bid * kRowPair);
   }
   for (int off = 1; off < L; off <<= 1) {
     fx += __shfl_xor(fx, off, kWave);
@@ -226,6 +227,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows)
       s.e_coul[i] = pec;
     }
   }
+}
+
+template <int MODE, bool THERMO, bool PERATOM, int SP, bool LTAB, bool F32>
+__global__ __launch_bounds__(kBlock) void k_pair_verlet(SysView s, double* rows) {
+  pair_verlet_body<MODE, THERMO, PERATOM, SP, LTAB, F32>(s, rows, (int)blockIdx.x);
 }
 
 template <int MODE, int SP, bool LTAB, bool F32>
@@ -409,6 +415,35 @@ void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, doub
     launch_pair_t<1, 4>(s, st, thermo, peratom, rows, grid);
   else
     launch_pair_t<1, 16>(s, st, thermo, peratom, rows, grid);
+}
+
+// ------------------------------------------------------------------ packs
+// thermo on, per-atom outputs off, LDS type tables (packs need ntypes <= kMaxTypes)
+
+template <int MODE, int SP, bool F32>
+__global__ __launch_bounds__(kBlock) void kp_pair(PACK_PARAMS) {
+  PACK_ENTRY(e, bid)
+  pair_verlet_body<MODE, true, false, SP, true, F32>(e.v, e.rows_pair, bid);
+}
+
+template <int MODE, int SP>
+void launch_kp_pair(const PackLaunch& L, int nblocks, int f32, hipStream_t st) {
+  if (f32)
+    hipLaunchKernelGGL((kp_pair<MODE, SP, true>), dim3(nblocks), dim3(kBlock), 0, st, PACK_ARGS(L));
+  else
+    hipLaunchKernelGGL((kp_pair<MODE, SP, false>), dim3(nblocks), dim3(kBlock), 0, st, PACK_ARGS(L));
+}
+
+void launch_pack_pair(const PackLaunch& L, int nblocks, const PackSig& g, hipStream_t st) {
+  if (nblocks <= 0) return;
+  if (g.pair_mode == 2)
+    launch_kp_pair<2, 0>(L, nblocks, g.f32, st);
+  else if (g.pair_mode == 0)
+    launch_kp_pair<0, 0>(L, nblocks, g.f32, st);
+  else if (g.pair_sp <= 4)
+    launch_kp_pair<1, 4>(L, nblocks, g.f32, st);
+  else
+    launch_kp_pair<1, 16>(L, nblocks, g.f32, st);
 }
 
 }  // namespace rbmd

@@ -25,7 +25,8 @@ EXPORTS = [
     "rbmd_rbe_set_kvectors", "rbmd_rbe_sample", "rbmd_rbe_sample_host", "rbmd_rbe_S",
     "rbmd_rbe_structure_factor", "rbmd_rbe_force", "rbmd_ewald_force", "rbmd_force_bonded", "rbmd_md_init",
     "rbmd_md_step", "rbmd_md_run_graph", "rbmd_md_enqueue", "rbmd_md_wait", "rbmd_md_query", "rbmd_md_thermo", "rbmd_md_energy", "rbmd_profile_enable",
-    "rbmd_profile_reset", "rbmd_profile_get", "rbmd_abi_sizeof",
+    "rbmd_profile_reset", "rbmd_profile_get", "rbmd_abi_sizeof", "rbmd_pack_create", "rbmd_pack_destroy",
+    "rbmd_pack_step", "rbmd_pack_enqueue", "rbmd_pack_wait",
 ]
 
 
@@ -83,6 +84,11 @@ def load():
     L.rbmd_profile_enable.argtypes = [vp, C.c_int32]
     L.rbmd_profile_reset.argtypes = [vp]
     L.rbmd_profile_get.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), pd]
+    L.rbmd_pack_create.argtypes = [vp, C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    L.rbmd_pack_destroy.argtypes = [vp]
+    L.rbmd_pack_step.argtypes = [vp, C.c_int64]
+    L.rbmd_pack_enqueue.argtypes = [vp, C.c_int64]
+    L.rbmd_pack_wait.argtypes = [vp, pi]
     L.rbmd_abi_sizeof.argtypes = [C.c_int32]
     L.rbmd_abi_sizeof.restype = C.c_int32
     for name in EXPORTS:
@@ -129,9 +135,12 @@ class Context:
         check(self.L.rbmd_ctx_create(int(device), C.byref(h)))
         self.h = h
         self._systems = weakref.WeakSet()
+        self._packs = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
+            for p in list(self._packs):
+                p.close()
             for s in list(self._systems):
                 s.close()
             self.L.rbmd_ctx_destroy(self.h)
@@ -323,3 +332,44 @@ class GpuSystem:
 
     def md_energy(self):
         return check(self.L.rbmd_md_energy(self.h))
+
+
+class Pack:
+    """Systems of one shape stepped together, one launch per kernel of the
+    step (rbmd_pack_*).  Each system's results are the same as stepping it
+    alone; the systems stay usable (thermo, state) through their GpuSystem."""
+
+    def __init__(self, ctx: Context, systems):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.systems = list(systems)
+        arr = (C.c_void_p * len(self.systems))(*[s.h.value for s in self.systems])
+        h = C.c_void_p()
+        check(self.L.rbmd_pack_create(ctx.h, arr, len(self.systems), C.byref(h)))
+        self.h = h
+        ctx._packs.add(self)
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            self.L.rbmd_pack_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n):
+        """n packed steps, launched kernel by kernel; waits and checks every system"""
+        check(self.L.rbmd_pack_step(self.h, int(n)))
+        return self.wait()
+
+    def enqueue(self, n):
+        """enqueue n graph-replayed packed steps and return at once"""
+        check(self.L.rbmd_pack_enqueue(self.h, int(n)))
+
+    def wait(self, raise_on_error=True):
+        """wait for the pack; returns the status word of every system"""
+        st = np.zeros(len(self.systems), np.int32)
+        rc = self.L.rbmd_pack_wait(self.h, _p(st, C.c_int32))
+        if rc != 0 and raise_on_error:
+            check(rc)
+        return st

@@ -1,18 +1,21 @@
 """Many independent systems on one GPU: the plain loop, the packed ensemble and
 the served job stream.
 
-Every system owns its device state for its whole life (HBM-resident).  In the
-packed and served modes each system is bound to one of K contexts (one HIP
-stream each, HIP maps them onto the device's hardware queues), its MD steps
-replay one captured step graph (`rbmd_md_enqueue`), and the host only
-enqueues: systems on different streams run concurrently, which fills the GPU
-with the small kernels of many small systems.  The plain loop is the
-reference's way of running them: one system at a time, one stream, steps
-enqueued kernel by kernel (`rbmd_md_step`).
+Every system owns its device state for its whole life (HBM-resident).  In
+the packed and served modes the systems of one kind (salt or water: the same
+force field, constraints and thermostat) form a pack (rbmd_pack_*): every
+kernel of the MD step is one launch over the blocks of all of them, replayed
+from one captured graph, so a step of many small systems costs one chain of
+launches instead of one chain each.  Each kind's pack runs on its own context
+(HIP stream), so the kinds overlap on the device.  The older mode
+("streams") binds each system to one of K contexts and replays its own step
+graph; it is kept for comparison.  The plain loop is the reference's way of
+running many systems: one at a time, one stream, steps enqueued kernel by
+kernel (rbmd_md_step).
 
 The reference has no counterpart for this module (its executioner runs one
-system, framework/executioner/Executioner.cpp:62-129); the per-system step is
-the same `ExecutionNVT::Solve` sequence as `rbmd_md_step`.
+system, framework/executioner/Executioner.cpp:62-129); each system's step is
+the same ExecutionNVT::Solve sequence as rbmd_md_step, with the same bits.
 """
 import dataclasses
 import math
@@ -22,7 +25,7 @@ from typing import List, Optional
 import numpy as np
 
 from .config import salt_config, water_config
-from .lib import Context, GpuSystem
+from .lib import Context, GpuSystem, Pack
 from .system import make_salt, make_water
 
 
@@ -115,69 +118,171 @@ class Pool:
             c.close()
 
 
+class PackGroup:
+    """The live systems of one kind on one context, stepped as one pack.  Each
+    round steps the pack by min(chunk, the smallest remaining step count), so
+    every member stops exactly at its own count; the pack is rebuilt whenever
+    members join or leave."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.members = []  # [key, GpuSystem, remaining steps]
+        self.pack = None
+        self.n = 0
+
+    def add(self, key, g, steps):
+        self.members.append([key, g, steps])
+        self._drop_pack()
+
+    def _drop_pack(self):
+        if self.pack is not None:
+            self.pack.close()
+            self.pack = None
+
+    def launch(self, chunk):
+        self.n = 0
+        if not self.members:
+            return
+        if self.pack is None:
+            self.pack = Pack(self.ctx, [m[1] for m in self.members])
+        self.n = min(chunk, min(m[2] for m in self.members))
+        self.pack.enqueue(self.n)
+
+    def finish(self):
+        """wait for the round; returns the members that reached their count"""
+        if not self.n:
+            return []
+        self.pack.wait()
+        done = []
+        for m in self.members:
+            m[2] -= self.n
+            if m[2] <= 0:
+                done.append(m)
+        if done:
+            self.members = [m for m in self.members if m[2] > 0]
+            self._drop_pack()
+        return done
+
+    def close(self):
+        self._drop_pack()
+
+
 def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: int = 0,
-               systems: Optional[list] = None, precision: str = "fp64") -> dict:
-    """All systems resident at once; every round enqueues `chunk` graph steps
-    of every unfinished system on its stream, then waits for the round.  Only
-    the stepping is timed."""
-    pool = Pool(streams, device)
-    live = []
-    for sp in specs:
-        s, r = build(sp, precision)
-        k = pool.pick(sp.atoms)
-        g = GpuSystem(pool.ctxs[k], s, r)
-        g.md_init()
-        live.append([sp, g, 0])
-    for c in pool.ctxs:
-        c.synchronize()
-    t0 = time.perf_counter()
-    while True:
-        active = [e for e in live if e[2] < e[0].steps]
-        if not active:
-            break
-        for e in active:
-            n = min(chunk, e[0].steps - e[2])
-            e[1].md_enqueue(n)
-            e[2] += n
-        for e in active:
-            e[1].md_wait()
-    seconds = time.perf_counter() - t0
+               systems: Optional[list] = None, precision: str = "fp64", mode: str = "pack") -> dict:
+    """All systems resident at once.  mode "pack": one pack per kind, each on
+    its own context, `chunk` graph-replayed packed steps per round; mode
+    "streams": systems spread over `streams` contexts, `chunk` steps of each
+    system's own step graph per round.  Only the stepping is timed."""
+    if mode not in ("pack", "streams"):
+        raise ValueError(mode)
+    built = []
+    if mode == "pack":
+        kinds = sorted({sp.kind for sp in specs})
+        ctxs = {k: Context(device) for k in kinds}
+        groups = {k: PackGroup(ctxs[k]) for k in kinds}
+        for i, sp in enumerate(specs):
+            s, r = build(sp, precision)
+            g = GpuSystem(ctxs[sp.kind], s, r)
+            g.md_init()
+            built.append(g)
+            groups[sp.kind].add(i, g, sp.steps)
+        for c in ctxs.values():
+            c.synchronize()
+        t0 = time.perf_counter()
+        while any(gr.members for gr in groups.values()):
+            for gr in groups.values():
+                gr.launch(chunk)
+            for gr in groups.values():
+                gr.finish()
+        seconds = time.perf_counter() - t0
+        for gr in groups.values():
+            gr.close()
+        holders = list(ctxs.values())
+    else:
+        pool = Pool(streams, device)
+        live = []
+        for sp in specs:
+            s, r = build(sp, precision)
+            k = pool.pick(sp.atoms)
+            g = GpuSystem(pool.ctxs[k], s, r)
+            g.md_init()
+            built.append(g)
+            live.append([sp, g, 0])
+        for c in pool.ctxs:
+            c.synchronize()
+        t0 = time.perf_counter()
+        while True:
+            active = [e for e in live if e[2] < e[0].steps]
+            if not active:
+                break
+            for e in active:
+                n = min(chunk, e[0].steps - e[2])
+                e[1].md_enqueue(n)
+                e[2] += n
+            for e in active:
+                e[1].md_wait()
+        seconds = time.perf_counter() - t0
+        holders = [pool]
     atom_steps = sum(sp.atoms * sp.steps for sp in specs)
     if systems is not None:
-        for e in live:
-            e[1]._pool = pool  # the contexts live as long as the returned systems
-        systems.extend(e[1] for e in live)
+        for g in built:
+            g._holders = holders  # the contexts live as long as the returned systems
+        systems.extend(built)
     else:
-        for e in live:
-            e[1].close()
-        pool.close()
+        for g in built:
+            g.close()
+        for h in holders:
+            h.close()
     return {"atom_steps": atom_steps, "seconds": seconds, "atom_steps_per_s": atom_steps / seconds,
-            "streams": streams, "chunk": chunk}
+            "mode": mode, "contexts": len(holders) if mode == "pack" else streams, "chunk": chunk}
 
 
 def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: float = 200e9, max_live: int = 64,
           device: int = 0, time_scale: float = 1.0, results: Optional[dict] = None,
-          prebuilt: Optional[dict] = None, precision: str = "fp64") -> dict:
+          prebuilt: Optional[dict] = None, precision: str = "fp64", mode: str = "pack") -> dict:
     """Continuous batching of a job stream.
 
     Jobs arrive at spec.arrival * time_scale seconds after the start.  A job
     is admitted when a slot (max_live) and HBM (footprint estimate against
-    hbm_budget) are free; admitted jobs are stepped `chunk` steps at a time on
-    their streams, interleaved with every other live job, and finish after
-    their step count.  Latency = finish - arrival.  Returns p50 / p99
-    latency and the aggregate atom-steps/s of the run.  `prebuilt` maps a
-    job index to its (System, RunConfig), i.e. the job's input as it
-    arrives; without it inputs are generated at admission.
+    hbm_budget) are free; admitted jobs are stepped `chunk` steps at a time,
+    interleaved with every other live job (mode "pack": in the pack of their
+    kind, rebuilt as jobs join and leave; mode "streams": each on one of
+    `streams` contexts with its own step graph), and finish after their step
+    count.  Latency = finish - arrival.  Returns p50 / p99 latency and the
+    aggregate atom-steps/s of the run.  `prebuilt` maps a job index to its
+    (System, RunConfig), i.e. the job's input as it arrives; without it
+    inputs are generated at admission.
     """
-    pool = Pool(streams, device)
+    if mode not in ("pack", "streams"):
+        raise ValueError(mode)
     pending = sorted(range(len(jobs)), key=lambda i: jobs[i].arrival)
     waiting: List[int] = []
-    live = {}
+    live = {}  # job -> [GpuSystem, steps done, context slot, footprint]
     used = 0.0
     lat = []
     done_atom_steps = 0
+    if mode == "pack":
+        kinds = sorted({j.kind for j in jobs})
+        ctxs = {k: Context(device) for k in kinds}
+        groups = {k: PackGroup(ctxs[k]) for k in kinds}
+        pool = None
+    else:
+        pool = Pool(streams, device)
     t_start = time.perf_counter()
     nxt = 0
+
+    def finish_job(j):
+        nonlocal used, done_atom_steps
+        g, _, k, need = live.pop(j)
+        lat.append((time.perf_counter() - t_start) - jobs[j].arrival * time_scale)
+        done_atom_steps += jobs[j].atoms * jobs[j].steps
+        if results is not None:
+            results[j] = g.state()
+        g.close()
+        if pool is not None:
+            pool.release(k, jobs[j].atoms)
+        used -= need
+
     while nxt < len(pending) or waiting or live:
         now = time.perf_counter() - t_start
         while nxt < len(pending) and jobs[pending[nxt]].arrival * time_scale <= now:
@@ -191,38 +296,49 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
             if live and used + need > hbm_budget:
                 break
             waiting.pop(0)
-            k = pool.pick(jobs[j].atoms)
-            g = GpuSystem(pool.ctxs[k], s, r)
+            if pool is not None:
+                k = pool.pick(jobs[j].atoms)
+                g = GpuSystem(pool.ctxs[k], s, r)
+            else:
+                k = jobs[j].kind
+                g = GpuSystem(ctxs[k], s, r)
             g.md_init()
             live[j] = [g, 0, k, need]
             used += need
+            if pool is None:
+                groups[k].add(j, g, jobs[j].steps)
         if not live:
             if nxt < len(pending):
                 time.sleep(max(0.0, min(0.01, jobs[pending[nxt]].arrival * time_scale - now)))
             continue
-        for j, e in live.items():
-            n = min(chunk, jobs[j].steps - e[1])
-            e[0].md_enqueue(n)
-            e[1] += n
-        finished = []
-        for j, e in live.items():
-            e[0].md_wait()
-            if e[1] >= jobs[j].steps:
-                finished.append(j)
-        for j in finished:
-            g, _, k, need = live.pop(j)
-            t_done = time.perf_counter() - t_start
-            lat.append(t_done - jobs[j].arrival * time_scale)
-            done_atom_steps += jobs[j].atoms * jobs[j].steps
-            if results is not None:
-                results[j] = g.state()
-            g.close()
-            pool.release(k, jobs[j].atoms)
-            used -= need
+        if pool is None:
+            for gr in groups.values():
+                gr.launch(chunk)
+            for gr in groups.values():
+                for key, _, _ in gr.finish():
+                    finish_job(key)
+        else:
+            for j, e in live.items():
+                n = min(chunk, jobs[j].steps - e[1])
+                e[0].md_enqueue(n)
+                e[1] += n
+            finished = []
+            for j, e in live.items():
+                e[0].md_wait()
+                if e[1] >= jobs[j].steps:
+                    finished.append(j)
+            for j in finished:
+                finish_job(j)
     wall = time.perf_counter() - t_start
-    pool.close()
+    if pool is not None:
+        pool.close()
+    else:
+        for gr in groups.values():
+            gr.close()
+        for c in ctxs.values():
+            c.close()
     lat = np.sort(np.asarray(lat))
     return {"jobs": len(jobs), "wall_s": wall, "atom_steps": done_atom_steps,
             "atom_steps_per_s": done_atom_steps / wall if wall > 0 else 0.0,
             "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
-            "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None}
+            "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None, "mode": mode}

@@ -1,5 +1,7 @@
 """Many systems on one GPU: every job's result is the same alone and in the
-mix (packed streams with graph steps and the served loop), bit for bit."""
+mix, bit for bit: the packed ensemble and the served loop, each with packs
+(one launch per kernel for all systems of a kind) and with per-system step
+graphs on several streams."""
 import numpy as np
 import pytest
 
@@ -22,11 +24,12 @@ def alone(ctx, spec, precision="fp64"):
     return out
 
 
+@pytest.mark.parametrize("mode", ["pack", "streams"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_packed_equals_alone(ctx, precision):
-    print(f"[case] packed ensemble vs alone ({precision})", flush=True)
+def test_packed_equals_alone(ctx, precision, mode):
+    print(f"[case] packed ensemble vs alone ({precision}, {mode})", flush=True)
     systems = []
-    run_packed(JOBS, streams=2, chunk=7, systems=systems, precision=precision)
+    run_packed(JOBS, streams=2, chunk=7, systems=systems, precision=precision, mode=mode)
     for spec, g in zip(JOBS, systems):
         ref = alone(ctx, spec, precision)
         got = g.state()
@@ -36,11 +39,12 @@ def test_packed_equals_alone(ctx, precision):
         g.close()
 
 
+@pytest.mark.parametrize("mode", ["pack", "streams"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_served_equals_alone(ctx, precision):
-    print(f"[case] served stream vs alone ({precision})", flush=True)
+def test_served_equals_alone(ctx, precision, mode):
+    print(f"[case] served stream vs alone ({precision}, {mode})", flush=True)
     results = {}
-    r = serve(JOBS, streams=3, chunk=9, results=results, time_scale=1.0, precision=precision)
+    r = serve(JOBS, streams=3, chunk=9, results=results, time_scale=1.0, precision=precision, mode=mode)
     assert r["jobs"] == len(JOBS) and r["p99_latency_s"] >= r["p50_latency_s"] > 0
     for j, spec in enumerate(JOBS):
         ref = alone(ctx, spec, precision)
