@@ -109,14 +109,21 @@ def cpu_baseline(system, run, steps, threads):
 def bench_ensemble(args):
     """Packed ensemble (graph steps on several streams) against the plain
     one-system-at-a-time FP64 loop on the same systems."""
-    from rbmd_amd.serve import ensemble_specs, run_packed, run_plain
+    from rbmd_amd.serve import build_all, ensemble_specs, run_packed, run_plain
 
     specs = ensemble_specs(args.systems, args.ens_steps, seed=2024, n_max=args.max_atoms)
     print(f"[bench] ensemble: {len(specs)} systems, {sum(s.atoms for s in specs)} atoms, "
           f"{args.ens_steps} steps each", file=sys.stderr, flush=True)
-    plain = run_plain(specs)  # the plain loop is FP64 by definition
+    say = lambda i: print(f"[bench] built {i} systems", file=sys.stderr, flush=True)
+    inputs = build_all(specs, "fp64", progress=say)
+    ran = lambda i: print(f"[bench] plain loop: {i} systems done", file=sys.stderr, flush=True)
+    plain = run_plain(specs, inputs=inputs, progress=ran)  # the plain loop is FP64 by definition
     print(f"[bench] plain loop {plain['atom_steps_per_s']:.4g} atom-steps/s", file=sys.stderr, flush=True)
-    packed = run_packed(specs, streams=args.streams, chunk=args.chunk, precision=args.precision, mode=args.ens_mode)
+    if args.precision != "fp64":
+        inputs = [(s, type(r)(**{**r.__dict__, "precision": args.precision})) for s, r in inputs]
+    packed = run_packed(specs, streams=args.streams, chunk=args.chunk, precision=args.precision, mode=args.ens_mode,
+                        inputs=inputs)
+    print(f"[bench] {args.ens_mode} {packed['atom_steps_per_s']:.4g} atom-steps/s", file=sys.stderr, flush=True)
     out = {"metric": "atom-steps/s (ensemble, aggregate)", "value": round(packed["atom_steps_per_s"], 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": args.ens_steps, "warmup": 0,
            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -139,9 +146,15 @@ def bench_serve(args):
     jobs = stream_specs(args.rate, args.duration, seed=77, n_max=args.max_atoms, steps_min=200,
                         steps_max=args.max_steps)
     print(f"[bench] serve: {len(jobs)} jobs over {args.duration} s", file=sys.stderr, flush=True)
-    inputs = {j: build(sp, args.precision) for j, sp in enumerate(jobs)}  # each job's input as it arrives
+    inputs = {}
+    for j, sp in enumerate(jobs):  # each job's input as it arrives
+        inputs[j] = build(sp, args.precision)
+        if (j + 1) % 16 == 0:
+            print(f"[bench] built {j + 1} jobs", file=sys.stderr, flush=True)
+    say = lambda n: print(f"[bench] served {n} jobs", file=sys.stderr, flush=True)
     r = serve(jobs, streams=args.streams, chunk=args.chunk, prebuilt=inputs, precision=args.precision,
-              mode=args.ens_mode)
+              mode=args.ens_mode, progress=say)
+    offered = sum(sp.atoms * sp.steps for sp in jobs) / args.duration
     out = {"metric": "served jobs: p50/p99 latency, aggregate atom-steps/s", "value": round(r["atom_steps_per_s"], 1),
            "unit": "atom-steps/s", "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype_of(args.precision),
@@ -152,6 +165,7 @@ def bench_serve(args):
                       "streams": args.streams if args.ens_mode == "streams" else None, "chunk": args.chunk,
                       "precision": args.precision, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
            "jobs": r["jobs"], "p50_latency_s": r["p50_latency_s"], "p99_latency_s": r["p99_latency_s"],
+           "offered_atom_steps_per_s": round(offered, 1),
            "wall_s": round(r["wall_s"], 3)}
     print(json.dumps(out))
 

@@ -71,20 +71,43 @@ def stream_specs(rate: float, duration: float, seed: int, n_min: int = 1000, n_m
                            int(rng.integers(steps_min, steps_max + 1)), int(rng.integers(1, 2 ** 31)), t))
 
 
+def water_rbe_p(atoms: int) -> int:
+    """RBE k vectors per step for a water job: 200 up to 30,000 atoms, then
+    growing with N.  At fixed P the per-step RBE force error grows like
+    sqrt(N) (0.34 / 1.09 / 3.3 of the force RMS at 3k / 30k / 300k atoms, P =
+    200); at 300k atoms and P = 200 the lattice start fails SHAKE within 10
+    steps, in the CPU oracle as on the GPU, while P = 200 N / 30,000 keeps the
+    30,000-atom error level."""
+    return max(200, int(round(200 * atoms / 30000)))
+
+
 def build(spec: JobSpec, precision: str = "fp64"):
     if spec.kind == "salt":
         return make_salt(n_ions=spec.atoms, seed=spec.seed), salt_config(precision=precision)
-    return make_water(n_mol=spec.atoms // 3, seed=spec.seed), water_config(precision=precision)
+    return (make_water(n_mol=spec.atoms // 3, seed=spec.seed),
+            water_config(precision=precision, rbe_p=water_rbe_p(spec.atoms)))
 
 
-def run_plain(specs: List[JobSpec], device: int = 0, precision: str = "fp64") -> dict:
+def build_all(specs: List[JobSpec], precision: str = "fp64", progress=None) -> list:
+    """every job's (System, RunConfig); progress(i) is called every 32 builds"""
+    out = []
+    for i, sp in enumerate(specs):
+        out.append(build(sp, precision))
+        if progress is not None and (i + 1) % 32 == 0:
+            progress(i + 1)
+    return out
+
+
+def run_plain(specs: List[JobSpec], device: int = 0, precision: str = "fp64", inputs: Optional[list] = None,
+              progress=None) -> dict:
     """One system at a time on one stream, eager steps (rbmd_md_step).  Only
-    the MD steps are timed (system set-up and md_init are not)."""
+    the MD steps are timed (system set-up and md_init are not).  inputs: the
+    jobs' (System, RunConfig) (built here when absent)."""
     ctx = Context(device)
     busy = 0.0
     atom_steps = 0
-    for sp in specs:
-        s, r = build(sp, precision)
+    for i, sp in enumerate(specs):
+        s, r = inputs[i] if inputs is not None else build(sp, precision)
         g = GpuSystem(ctx, s, r)
         g.md_init()
         ctx.synchronize()
@@ -94,6 +117,8 @@ def run_plain(specs: List[JobSpec], device: int = 0, precision: str = "fp64") ->
         busy += time.perf_counter() - t0
         atom_steps += sp.atoms * sp.steps
         g.close()
+        if progress is not None and (i + 1) % 64 == 0:
+            progress(i + 1)
     ctx.close()
     return {"atom_steps": atom_steps, "seconds": busy, "atom_steps_per_s": atom_steps / busy}
 
@@ -168,7 +193,8 @@ class PackGroup:
 
 
 def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: int = 0,
-               systems: Optional[list] = None, precision: str = "fp64", mode: str = "pack") -> dict:
+               systems: Optional[list] = None, precision: str = "fp64", mode: str = "pack",
+               inputs: Optional[list] = None) -> dict:
     """All systems resident at once.  mode "pack": one pack per kind, each on
     its own context, `chunk` graph-replayed packed steps per round; mode
     "streams": systems spread over `streams` contexts, `chunk` steps of each
@@ -181,7 +207,7 @@ def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: 
         ctxs = {k: Context(device) for k in kinds}
         groups = {k: PackGroup(ctxs[k]) for k in kinds}
         for i, sp in enumerate(specs):
-            s, r = build(sp, precision)
+            s, r = inputs[i] if inputs is not None else build(sp, precision)
             g = GpuSystem(ctxs[sp.kind], s, r)
             g.md_init()
             built.append(g)
@@ -201,8 +227,8 @@ def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: 
     else:
         pool = Pool(streams, device)
         live = []
-        for sp in specs:
-            s, r = build(sp, precision)
+        for i, sp in enumerate(specs):
+            s, r = inputs[i] if inputs is not None else build(sp, precision)
             k = pool.pick(sp.atoms)
             g = GpuSystem(pool.ctxs[k], s, r)
             g.md_init()
@@ -239,7 +265,7 @@ def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: 
 
 def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: float = 200e9, max_live: int = 64,
           device: int = 0, time_scale: float = 1.0, results: Optional[dict] = None,
-          prebuilt: Optional[dict] = None, precision: str = "fp64", mode: str = "pack") -> dict:
+          prebuilt: Optional[dict] = None, precision: str = "fp64", mode: str = "pack", progress=None) -> dict:
     """Continuous batching of a job stream.
 
     Jobs arrive at spec.arrival * time_scale seconds after the start.  A job
@@ -282,6 +308,8 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
         if pool is not None:
             pool.release(k, jobs[j].atoms)
         used -= need
+        if progress is not None and len(lat) % 10 == 0:
+            progress(len(lat))
 
     while nxt < len(pending) or waiting or live:
         now = time.perf_counter() - t_start
