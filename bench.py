@@ -73,6 +73,14 @@ def workload(name, precision, rbe_p=None):
     return s, r, desc, dt_fs
 
 
+def tile_choice(run, n_atoms):
+    """the tile choices the system was created with (0: the library default)"""
+    from rbmd_amd.abi import make_params
+
+    p = make_params(run, n_atoms)
+    return {"pair_lanes": p.pair_lanes, "kforce_lanes": p.kforce_lanes, "rho_kblock": p.rho_kblock}
+
+
 def dtype_of(precision):
     return "f64" if precision == "fp64" else "f32 pair scalar and sin/cos, f64 vectors and sums"
 
@@ -301,7 +309,7 @@ def main():
         "dtype": dtype_of(args.precision),
         "data": "synthetic",
         "config": {"workload": desc, "atoms": system.n_atoms, "precision": args.precision, "rbe_p": P,
-                   "cut_off": run.cut_off},
+                   "cut_off": run.cut_off, "tiles": tile_choice(run, system.n_atoms)},
         "ns_per_day": round((args.steps / elapsed) * dt_fs * 86400 / 1e6, 3) if dt_fs else None,
         "roofline": roof,
         "kernels_us": {k: round(v["mean_us"], 2) for k, v in kernels.items()},

@@ -84,6 +84,9 @@ class Params(C.Structure):
         ("energy_every", C.c_int32),
         ("rbl_rho", C.c_double),
         ("pair_path", C.c_int32),
+        ("pair_lanes", C.c_int32),
+        ("kforce_lanes", C.c_int32),
+        ("rho_kblock", C.c_int32),
     ]
 
 
@@ -173,8 +176,10 @@ class DescHolder:
         self.desc = d
 
 
-def make_params(run):
-    """Params from a RunConfig (rbmd_amd.config)."""
+def make_params(run, n_atoms=None):
+    """Params from a RunConfig (rbmd_amd.config).  Tile choices: the
+    RunConfig's explicit values, else (run.tiles == "tuned" and n_atoms given)
+    the tuned table rbmd_amd/tiles.json, else the library's defaults (0)."""
     p = Params()
     p.unit = UNIT[run.unit]
     p.force_field = FORCE_FIELD[run.force_field]
@@ -201,4 +206,12 @@ def make_params(run):
     p.energy_every = int(run.energy_every)
     p.rbl_rho = float(run.rbl_rho)
     p.pair_path = PAIR_PATH[run.pair_path]
+    tiles = {"pair_lanes": 0, "kforce_lanes": 0, "rho_kblock": 0}
+    if getattr(run, "tiles", "tuned") == "tuned" and n_atoms is not None:
+        from .tiles import choose
+
+        tiles.update(choose(run.force_field, run.precision, int(n_atoms)))
+    for k in tiles:
+        v = int(getattr(run, k, 0) or 0)
+        setattr(p, k, v if v else int(tiles[k]))
     return p

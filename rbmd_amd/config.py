@@ -43,6 +43,10 @@ class RunConfig:
     energy_every: int = 0
     rbl_rho: float = 0.0
     pair_path: str = "lists"
+    tiles: str = "tuned"  # "tuned": rbmd_amd/tiles.json by kind, precision and size; "default": library defaults
+    pair_lanes: int = 0  # explicit tile choices (0: from `tiles`)
+    kforce_lanes: int = 0
+    rho_kblock: int = 0
     thermo_interval: int = 0
 
 
@@ -152,6 +156,9 @@ def run_config_from_json(doc: dict) -> RunConfig:
     rc.energy_every = int(ext2.get("energy_every", rc.thermo_interval))
     rc.rbl_rho = float(ext2.get("rbl_rho", 0.0))
     rc.pair_path = str(ext2.get("pair_path", "lists"))
+    rc.tiles = str(ext2.get("tiles", "tuned"))
+    for key in ("pair_lanes", "kforce_lanes", "rho_kblock"):
+        setattr(rc, key, int(ext2.get(key, 0)))
     return rc
 
 

@@ -705,6 +705,12 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     if (!(d->mass[i] > 0)) return fail(RBMD_ERR_ARG, "atom %d has non-positive mass", i);
   }
   if (p->pair_path != RBMD_PAIR_CELLS && p->pair_path != RBMD_PAIR_LISTS) return fail(RBMD_ERR_ARG, "bad pair_path");
+  if (p->pair_lanes != 0 && p->pair_lanes != 4 && p->pair_lanes != 8 && p->pair_lanes != 16)
+    return fail(RBMD_ERR_ARG, "pair_lanes must be 0, 4, 8 or 16");
+  if (p->kforce_lanes != 0 && p->kforce_lanes != 2 && p->kforce_lanes != 4 && p->kforce_lanes != 8)
+    return fail(RBMD_ERR_ARG, "kforce_lanes must be 0, 2, 4 or 8");
+  if (p->rho_kblock != 0 && p->rho_kblock != 4 && p->rho_kblock != 8 && p->rho_kblock != 16)
+    return fail(RBMD_ERR_ARG, "rho_kblock must be 0, 4, 8 or 16");
   if (p->coulomb_type == RBMD_COULOMB_RBE && p->rbe_p <= 0) return fail(RBMD_ERR_ARG, "coulomb_sample_num must be > 0");
   if (p->coulomb_type != RBMD_COULOMB_NONE && !(p->alpha > 0)) return fail(RBMD_ERR_ARG, "alpha must be > 0");
   if (d->special_offsets && d->special_ids && d->special_weights) {
@@ -756,10 +762,9 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   // lanes per atom in the pair kernel: enough threads to fill 256 CUs for
   // small systems, fewer lanes (longer per-lane loops) for large ones
   v.lanes_lg = n <= 60000 ? 4 : (n <= 250000 ? 3 : 2);
-  if (const char* e = std::getenv("RBMD_PAIR_LANES_LG")) {
-    int lg = std::atoi(e);
-    if (lg >= 2 && lg <= 4) v.lanes_lg = lg;
-  }
+  if (p->pair_lanes) v.lanes_lg = p->pair_lanes == 4 ? 2 : (p->pair_lanes == 8 ? 3 : 4);
+  v.k_lanes_lg = p->kforce_lanes ? (p->kforce_lanes == 2 ? 1 : (p->kforce_lanes == 8 ? 3 : 2)) : 2;
+  v.rho_kb = p->rho_kblock ? p->rho_kblock : 8;
   v.sp_max = 0;
   if (d->special_offsets && d->special_ids && d->special_weights)
     for (int i = 0; i < n; ++i) v.sp_max = std::max(v.sp_max, d->special_offsets[i + 1] - d->special_offsets[i]);
@@ -1781,6 +1786,8 @@ rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n,
         s->p.coulomb_type != s0->p.coulomb_type || has_bonded(s) != has_bonded(s0) || shake_on(s) != shake_on(s0) ||
         tc != s0->p.temp_ctrl)
       return fail(RBMD_ERR_ARG, "system %d differs from system 0 in force field, precision, constraints or thermostat", k);
+    if (s->v.k_lanes_lg != s0->v.k_lanes_lg || s->v.rho_kb != s0->v.rho_kb)
+      return fail(RBMD_ERR_ARG, "system %d differs from system 0 in its k-space tile choices", k);
     for (int j = 0; j < k; ++j)
       if (systems[j] == s) return fail(RBMD_ERR_ARG, "system %d appears twice", k);
   }
@@ -1801,6 +1808,8 @@ rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n,
   p->sig.has_k = p->has_k;
   p->sig.ewald = s0->p.coulomb_type == RBMD_COULOMB_EWALD;
   p->sig.has_b = p->has_b;
+  p->sig.k_lanes_lg = s0->v.k_lanes_lg;
+  p->sig.rho_kb = s0->v.rho_kb;
   auto dalloc = [&](void** q, size_t bytes) -> rbmd_status {
     hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 4));
     if (e != hipSuccess) return fail(RBMD_ERR_OOM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));

@@ -37,6 +37,8 @@ struct SysView {
   int nseg;         // VERLET: one list segment per (x, y) column of the 3x3x3 stencil (9); RBL: 1
   int seg_cap;      // slots per segment (multiple of the lane count)
   int lanes_lg;     // log2 of the lanes per atom in the pair kernel (2..4)
+  int k_lanes_lg;   // log2 of the lanes per atom in the k-space force kernel (1..3)
+  int rho_kb;       // k vectors per workgroup in the structure-factor kernel (4, 8, 16)
   int sp_max;       // longest special list
   int n_items;      // capacity of the work-item list
   int weighted;     // special-bond weights in the pair term
@@ -270,7 +272,7 @@ enum PackKernel {
 int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel);
 
 struct PackSig {  // launch-wide choices, equal for every system of a pack
-  int pair_mode, pair_sp, f32, has_k, ewald, has_b;
+  int pair_mode, pair_sp, f32, has_k, ewald, has_b, k_lanes_lg, rho_kb;
 };
 
 // Packed launches of one kernel (nblocks = total over the pack; lds = the

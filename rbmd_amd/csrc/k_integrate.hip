@@ -509,6 +509,7 @@ void launch_scale_v(const SysView& s, hipStream_t st, const double* scale) {
 
 
 __device__ void nhc_integrate(NhcState& S, const NhcParams& p) {
+#pragma clang fp contract(off)
   double T = S.tempT;
   double kecur = p.tdof * p.boltz * T;
   S.eta_mass[0] = p.tdof * p.boltz * S.t_target / (p.t_freq * p.t_freq);
@@ -546,6 +547,7 @@ __device__ void nhc_integrate(NhcState& S, const NhcParams& p) {
 }
 
 __device__ void temp_target(NhcState& S, const NhcParams& p) {
+#pragma clang fp contract(off)
   if (p.t_stop == p.t_start) {
     S.t_target = p.t_start;
   } else {
@@ -557,6 +559,7 @@ __device__ void temp_target(NhcState& S, const NhcParams& p) {
 }
 
 __device__ void compute_tempe(NhcState& S, const NhcParams& p, double ke_sum) {
+#pragma clang fp contract(off)
   S.tempT_sum = ke_sum;
   double n = p.n;
   if (p.inbuild)
@@ -570,6 +573,7 @@ __device__ void compute_tempe(NhcState& S, const NhcParams& p, double ke_sum) {
 }
 
 __device__ void compute_pressure(NhcState& S, const NhcParams& p, const double* sums) {
+#pragma clang fp contract(off)
   for (int k = 0; k < 6; ++k) {
     double w = sums[S_VIR_NEAR + k];
     if (!p.lj_only) w += sums[S_VIR_K + k] * p.qqr2e;
@@ -587,10 +591,12 @@ __device__ void compute_pressure(NhcState& S, const NhcParams& p, const double* 
   S.e_angle = p.na > 0 ? sums[S_EANGLE] / p.na : 0.0;
 }
 
-// Thread 0's thermostat step from the reduced sums.  Not inlined: the
-// single-system and packed NHC kernels run this one instruction sequence, so
-// their chain variables agree to the bit.
-__device__ __noinline__ void nhc_update(NhcState* state, const NhcParams& p, const double* sums, int phase) {
+// Thread 0's thermostat step from the reduced sums.  The thermostat
+// functions compile with FMA contraction off, so every kernel they are
+// inlined into (single-system and packed) evaluates the same separately
+// rounded operations and the chain variables agree to the bit.
+__device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, const double* sums, int phase) {
+#pragma clang fp contract(off)
   NhcState S = *state;
   switch (phase) {
     case 3:  // SetUp: ComputeTempe, ComputeTempTarget, chain masses (ExecutionNVT.cpp:1106-1122)

@@ -20,7 +20,7 @@
 namespace rbmd {
 
 constexpr int kRhoAtomsPerThread = 4;
-constexpr int kRhoK = 8;  // k-vectors per workgroup in the rho pass
+// k-vectors per workgroup in the rho pass: a tile choice (4, 8 or 16, SysView::rho_kb)
 
 // e^{i k.r} with k = 2 pi m / L is e^{2 pi i t} with the phase t = m.(r/L) in
 // turns (several hundred radians at |m| ~ 30): sincos_turns reduces it
@@ -46,9 +46,10 @@ __device__ __forceinline__ void phase_sincos(double mx, double my, double mz, do
 // reciprocal rounds to the same integer as the quotient
 __device__ __forceinline__ double k_to_m(double k, double box) { return rint(k * box * (0.5 / kPi)); }
 
-// bk: block of kRhoK k vectors, ba: block of kBlock * kRhoAtomsPerThread atoms
-template <bool F32>
+// bk: block of KB k vectors, ba: block of kBlock * kRhoAtomsPerThread atoms
+template <bool F32, int KB>
 __device__ __forceinline__ void rho_partial_body(const SysView& s, const KView& kv, const int bk, const int ba) {
+  constexpr int kRhoK = KB;
   __shared__ double lds[kBlock / kWave][2 * kRhoK];
   const int l0 = bk * kRhoK;
   double re[kRhoK], im[kRhoK];
@@ -103,10 +104,24 @@ __device__ __forceinline__ void rho_partial_body(const SysView& s, const KView& 
 }
 
 // k-vector blocks on x (a converged Ewald grid has millions), atom blocks on y
-template <bool F32>
+template <bool F32, int KB>
 __global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
-  rho_partial_body<F32>(s, kv, (int)blockIdx.x, (int)blockIdx.y);
+  rho_partial_body<F32, KB>(s, kv, (int)blockIdx.x, (int)blockIdx.y);
 }
+
+int rho_kblocks(const SysView& s, int P) { return (P + s.rho_kb - 1) / s.rho_kb; }
+
+// instantiate F(F32, KB) for the tile choice kb (4, 8 or 16)
+#define RHO_DISPATCH(kb, f32, F)                 \
+  do {                                           \
+    if ((kb) == 4) {                             \
+      if (f32) F(true, 4); else F(false, 4);     \
+    } else if ((kb) == 16) {                     \
+      if (f32) F(true, 16); else F(false, 16);   \
+    } else {                                     \
+      if (f32) F(true, 8); else F(false, 8);     \
+    }                                            \
+  } while (0)
 
 __device__ __forceinline__ void rho_reduce_body(const KView& kv, const int bid) {
   int t = bid * blockDim.x + threadIdx.x;
@@ -139,11 +154,10 @@ void launch_kvec_slot(double* dst, const double* ring, int R, int per, const int
 }
 
 void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st) {
-  dim3 grid((k.P + kRhoK - 1) / kRhoK, k.nblk_rho);
-  if (k.f32)
-    hipLaunchKernelGGL(k_rho_partial<true>, grid, dim3(kBlock), 0, st, s, k);
-  else
-    hipLaunchKernelGGL(k_rho_partial<false>, grid, dim3(kBlock), 0, st, s, k);
+  dim3 grid(rho_kblocks(s, k.P), k.nblk_rho);
+#define RHO_LAUNCH(F32_, KB_) hipLaunchKernelGGL((k_rho_partial<F32_, KB_>), grid, dim3(kBlock), 0, st, s, k)
+  RHO_DISPATCH(s.rho_kb, k.f32, RHO_LAUNCH);
+#undef RHO_LAUNCH
 }
 
 void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st) {
@@ -151,12 +165,13 @@ void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st) {
 }
 
 constexpr int kForceTile = 256;
-constexpr int kKLg = 2;  // 4 lanes per atom split the k list
 
-// prefactor pre_l = -4 pi / (V k^2) [* e^{-k^2/4alpha} for Ewald]; lane `sub`
-// of an atom takes k-vectors sub, sub + 4, ... of each LDS tile
-template <bool THERMO, bool PERATOM, bool F32>
+// prefactor pre_l = -4 pi / (V k^2) [* e^{-k^2/4alpha} for Ewald]; the 2^LG
+// lanes of an atom (a tile choice, SysView::k_lanes_lg) split each LDS tile:
+// lane `sub` takes k-vectors sub, sub + 2^LG, ...
+template <bool THERMO, bool PERATOM, bool F32, int LG>
 __device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, int ewald, double alpha, double* rows, const int bid) {
+  constexpr int kKLg = LG;
   constexpr int L = 1 << kKLg;
   __shared__ double tk[kForceTile][4];
   __shared__ double tm[kForceTile][3];
@@ -237,12 +252,23 @@ __device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, i
   }
 }
 
-template <bool THERMO, bool PERATOM, bool F32>
+template <bool THERMO, bool PERATOM, bool F32, int LG>
 __global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewald, double alpha, double* rows) {
-  kforce_body<THERMO, PERATOM, F32>(s, kv, ewald, alpha, rows, (int)blockIdx.x);
+  kforce_body<THERMO, PERATOM, F32, LG>(s, kv, ewald, alpha, rows, (int)blockIdx.x);
 }
 
-int kforce_blocks(const SysView& s) { return (int)((((size_t)s.n << kKLg) + kBlock - 1) / kBlock); }
+int kforce_blocks(const SysView& s) { return (int)((((size_t)s.n << s.k_lanes_lg) + kBlock - 1) / kBlock); }
+
+template <bool THERMO, bool PERATOM, bool F32>
+void launch_kforce_lg(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, double* rows,
+                      int grid) {
+  if (s.k_lanes_lg == 1)
+    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 1>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+  else if (s.k_lanes_lg == 3)
+    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 3>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+  else
+    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 2>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+}
 
 void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, int thermo, int peratom,
                    double* rows, int* nblocks) {
@@ -250,19 +276,19 @@ void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, 
   if (nblocks) *nblocks = grid;
   if (k.f32) {
     if (peratom)
-      hipLaunchKernelGGL((k_kforce<true, true, true>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+      launch_kforce_lg<true, true, true>(s, k, st, ewald, alpha, rows, grid);
     else if (thermo)
-      hipLaunchKernelGGL((k_kforce<true, false, true>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+      launch_kforce_lg<true, false, true>(s, k, st, ewald, alpha, rows, grid);
     else
-      hipLaunchKernelGGL((k_kforce<false, false, true>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+      launch_kforce_lg<false, false, true>(s, k, st, ewald, alpha, rows, grid);
     return;
   }
   if (peratom)
-    hipLaunchKernelGGL((k_kforce<true, true, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    launch_kforce_lg<true, true, false>(s, k, st, ewald, alpha, rows, grid);
   else if (thermo)
-    hipLaunchKernelGGL((k_kforce<true, false, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    launch_kforce_lg<true, false, false>(s, k, st, ewald, alpha, rows, grid);
   else
-    hipLaunchKernelGGL((k_kforce<false, false, false>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    launch_kforce_lg<false, false, false>(s, k, st, ewald, alpha, rows, grid);
 }
 
 // ------------------------------------------------------------------ packs
@@ -272,11 +298,11 @@ __global__ __launch_bounds__(kBlock) void kp_kvec_slot(PACK_PARAMS) {
   kvec_slot_body(const_cast<double*>(e.kv.kvec), e.kring, e.ring_R, 3 * e.kv.P, &e.nhc->step, bid, PACK_GRID());
 }
 
-template <bool F32>
+template <bool F32, int KB>
 __global__ __launch_bounds__(kBlock) void kp_rho_partial(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
-  const int kb = (e.kv.P + kRhoK - 1) / kRhoK;
-  rho_partial_body<F32>(e.v, e.kv, bid % kb, bid / kb);
+  const int kb = (e.kv.P + KB - 1) / KB;
+  rho_partial_body<F32, KB>(e.v, e.kv, bid % kb, bid / kb);
 }
 
 __global__ __launch_bounds__(kBlock) void kp_rho_reduce(PACK_PARAMS) {
@@ -284,10 +310,10 @@ __global__ __launch_bounds__(kBlock) void kp_rho_reduce(PACK_PARAMS) {
   rho_reduce_body(e.kv, bid);
 }
 
-template <bool F32>
+template <bool F32, int LG>
 __global__ __launch_bounds__(kBlock) void kp_kforce(PACK_PARAMS, int ewald) {
   PACK_ENTRY(e, bid)
-  kforce_body<true, false, F32>(e.v, e.kv, ewald, e.v.alpha, e.rows_k, bid);
+  kforce_body<true, false, F32, LG>(e.v, e.kv, ewald, e.v.alpha, e.rows_k, bid);
 }
 
 void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const PackSig& g, hipStream_t st) {
@@ -298,19 +324,24 @@ void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const Pack
       hipLaunchKernelGGL(kp_kvec_slot, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
       break;
     case PK_RHO:
-      if (g.f32)
-        hipLaunchKernelGGL(kp_rho_partial<true>, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
-      else
-        hipLaunchKernelGGL(kp_rho_partial<false>, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+#define KP_RHO_LAUNCH(F32_, KB_) hipLaunchKernelGGL((kp_rho_partial<F32_, KB_>), grid, dim3(kBlock), 0, st, PACK_ARGS(L))
+      RHO_DISPATCH(g.rho_kb, g.f32, KP_RHO_LAUNCH);
+#undef KP_RHO_LAUNCH
       break;
     case PK_RHO_RED:
       hipLaunchKernelGGL(kp_rho_reduce, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
       break;
     case PK_KFORCE:
-      if (g.f32)
-        hipLaunchKernelGGL(kp_kforce<true>, grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald);
-      else
-        hipLaunchKernelGGL(kp_kforce<false>, grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald);
+#define KP_KF_LAUNCH(F32_, LG_) \
+  hipLaunchKernelGGL((kp_kforce<F32_, LG_>), grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald)
+      if (g.k_lanes_lg == 1) {
+        if (g.f32) KP_KF_LAUNCH(true, 1); else KP_KF_LAUNCH(false, 1);
+      } else if (g.k_lanes_lg == 3) {
+        if (g.f32) KP_KF_LAUNCH(true, 3); else KP_KF_LAUNCH(false, 3);
+      } else {
+        if (g.f32) KP_KF_LAUNCH(true, 2); else KP_KF_LAUNCH(false, 2);
+      }
+#undef KP_KF_LAUNCH
       break;
     default:
       break;
@@ -346,7 +377,7 @@ int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel) {
     case PK_PAIR:
       return pair_blocks(s);
     case PK_RHO:
-      return ((P + kRhoK - 1) / kRhoK) * ((s.n + kBlock * kRhoAtomsPerThread - 1) / (kBlock * kRhoAtomsPerThread));
+      return rho_kblocks(s, P) * ((s.n + kBlock * kRhoAtomsPerThread - 1) / (kBlock * kRhoAtomsPerThread));
     case PK_RHO_RED:
       return (2 * P + kBlock - 1) / kBlock;
     case PK_KFORCE:

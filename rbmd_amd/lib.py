@@ -180,7 +180,7 @@ class GpuSystem:
         self.L = ctx.L
         self.n = system.n_atoms
         self._desc = DescHolder(system)
-        self._params = make_params(run)
+        self._params = make_params(run, system.n_atoms)
         h = C.c_void_p()
         check(self.L.rbmd_sys_create(ctx.h, C.byref(self._desc.desc), C.byref(self._params), C.byref(h)))
         self.h = h

@@ -101,3 +101,28 @@ def test_pack_rejects_mixed_shapes(ctx):
         Pack(ctx, [ga, gw])
     with pytest.raises(RbmdError):
         Pack(ctx, [ga, ga])
+
+
+def test_tile_options_agree(ctx):
+    """every tile option gives the same forces up to summation order"""
+    s = water(nmol=600, seed=5)
+    ref = None
+    for i, (pl, kl, kb) in enumerate([(4, 2, 4), (8, 4, 8), (16, 8, 16)]):
+        print(f"[case] tiles pair {pl} kforce {kl} rho {kb}", flush=True)
+        g = GpuSystem(ctx, s, water_run(pair_lanes=pl, kforce_lanes=kl, rho_kblock=kb))
+        g.md_init()
+        f = g.state()[2]
+        g.close()
+        if ref is None:
+            ref = f
+        else:
+            assert np.abs(f - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_pack_rejects_different_k_tiles(ctx):
+    a = GpuSystem(ctx, salt(n=600, seed=21), salt_run(kforce_lanes=2))
+    b = GpuSystem(ctx, salt(n=800, seed=22), salt_run(kforce_lanes=8))
+    a.md_init()
+    b.md_init()
+    with pytest.raises(RbmdError):
+        Pack(ctx, [a, b])
