@@ -1030,6 +1030,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   np.lj_only = v.lj_only;
   np.cvff = p->force_field == RBMD_FF_CVFF;
   np.shake_vir = p->fix_shake == RBMD_SHAKE_TRUE && s->na > 0;
+  np.shake_iter = s->shake_iter;
   NhcState zero;
   std::memset(&zero, 0, sizeof(zero));
   hipError_t e = hipMemcpy(s->nhc, &zero, sizeof(zero), hipMemcpyHostToDevice);
@@ -1493,7 +1494,6 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring) {
   c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, 1); });
   build_cells(s, false);
   if (shake) {
-    launch_fill_int(s->shake_iter, 0, 1, st);
     c->run("shake", [&] { launch_shake(s->v, s->iv, st, s->seg_rows(SEG_SHAKE), s->shake_iter); });
     s->seg_used[SEG_SHAKE] = (s->nmol + kBlock - 1) / kBlock;
     if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 1);
@@ -1737,10 +1737,7 @@ static void pack_enqueue_step(rbmd_pack* p) {
   cells(PK_SCAN, 0);
   cells(PK_SCATTER, p->lds_scatter);
   cells(PK_CELL_SORT, 0);
-  if (p->shake) {
-    integ(PK_ONE, 0, 0);
-    integ(PK_SHAKE, 0, 0);
-  }
+  if (p->shake) integ(PK_SHAKE, 0, 0);
   cells(PK_GATHER, 0);
   cells(PK_NBR, 0);
   c->run(kPackName[PK_PAIR], [&] { launch_pack_pair(p->L[PK_PAIR], p->nblk[PK_PAIR], p->sig, st); });

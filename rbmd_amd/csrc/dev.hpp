@@ -211,6 +211,7 @@ struct NhcParams {
   double volume_range;
   int64_t num_steps;
   int lj_only, cvff, shake_vir;
+  int* shake_iter;       // SHAKE iteration maximum of the step, zeroed at the step start (phases 1, 5)
 };
 // phase 1: step++ and NHC half step (InitialIntegrate); 2: ComputeTempe + pressure + NHC
 // (FinalIntegrate); 3: SetUp; 4: Berendsen/rescale end of step; 5: step++ only

@@ -611,6 +611,7 @@ __device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, 
       break;
     case 1:  // InitialIntegrate: TakeStep increments the step, then ComputeTempTarget + NHC
       S.step += 1;
+      if (p.shake_iter) *p.shake_iter = 0;
       if (p.mode == 0) {
         temp_target(S, p);
         nhc_integrate(S, p);
@@ -639,6 +640,7 @@ __device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, 
     }
     case 5:  // Berendsen-path step start: only the step counter
       S.step += 1;
+      if (p.shake_iter) *p.shake_iter = 0;
       S.factor = 1.0;
       break;
     default:

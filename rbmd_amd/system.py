@@ -160,6 +160,34 @@ def make_salt(n_ions: int = 4000, density: float = 0.5, seed: int = 1, temperatu
                   eps=np.array([1.0, 1.0]), sigma=np.array([1.0, 1.0]), kind="salt")
 
 
+def make_inbuild(dims, x_range, y_range, z_range, velocity_type: str = "GAUSS", seed: int = 1) -> System:
+    """The reference's in-build LJ lattice (MeshFreeCondition + LJInitCondition,
+    framework/InitCondition/MeshFreeCondition.cpp:45-100, LJInitCondition.cpp:37-99):
+    one particle at the centroid of every cell of a uniform dims[0] x dims[1] x
+    dims[2] grid over the ranges (x fastest, as VTK-m orders structured cells),
+    mass 1, one type with eps = sigma = 1, no charge; each atom's special list
+    is itself with weight 1.  GAUSS velocities are standard normal per
+    component (the reference draws them from an unseeded generator; here from
+    `seed`); ZERO gives zero velocities."""
+    dims = [int(d) for d in dims]
+    lo = np.array([x_range[0], y_range[0], z_range[0]], np.float64)
+    hi = np.array([x_range[1], y_range[1], z_range[1]], np.float64)
+    space = (hi - lo) / np.asarray(dims, np.float64)
+    k, j, i = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+    cells = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).astype(np.float64)
+    x = lo + (cells + 0.5) * space
+    n = len(x)
+    if velocity_type == "GAUSS":
+        v = np.random.default_rng(seed).standard_normal((n, 3))
+    else:
+        v = np.zeros((n, 3))
+    ids = np.arange(n, dtype=np.int32)
+    return System(lo=lo, hi=hi, x=x, v=v, type=np.zeros(n, np.int32), charge=np.zeros(n), mass=np.ones(n),
+                  eps=np.array([1.0]), sigma=np.array([1.0]), mol_id=ids.copy(),
+                  special_offsets=np.arange(n + 1, dtype=np.int32), special_ids=ids,
+                  special_weights=np.ones(n), kind="inbuild")
+
+
 def water_geometry(rng: np.random.Generator, n: int, bond: float = 1.0, angle_deg: float = 109.47):
     """Random orientations of an H-O-H unit: returns H1, H2 offsets from O, [n,3] each."""
     half = math.radians(angle_deg) / 2.0
