@@ -78,6 +78,9 @@ struct ProfRec {
 struct rbmd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream of a step: the k-space and bonded forces run on it beside
+  // the list build and pair forces (both default priority)
+  hipStream_t side = nullptr;
   bool prof = false;
   std::vector<ProfRec> recs;
   std::map<std::string, int> index;
@@ -175,6 +178,7 @@ struct rbmd_sys {
   double* kring = nullptr; // [2R][3P]
   double* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // side-stream branch of a step
   int ring_R = 256;
   int export_cap = 0;
   int64_t ring_batch = -1;  // batch of steps currently staged in half (batch % 2)
@@ -455,9 +459,10 @@ rbmd_status ensure_rho(rbmd_sys* s, int P) {
 
 // Full force evaluation on the current locator (ExecutionNVT::ComputeAllForce,
 // ExecutionNVT.cpp:195-235).  kvec: device pointer to this step's k vectors.
-void compute_force(rbmd_sys* s, const double* kvec, int thermo) {
+// k-space and bonded forces on stream st (they read the unsorted positions
+// and charges only, and write their own force arrays and thermo rows)
+static void launch_k_bonded(rbmd_sys* s, const double* kvec, int thermo, hipStream_t st) {
   rbmd_ctx* c = s->ctx;
-  launch_real(s, thermo, 0);
   if (has_k(s)) {
     KView k = s->kv;
     k.P = s->P;
@@ -465,19 +470,22 @@ void compute_force(rbmd_sys* s, const double* kvec, int thermo) {
     k.S_over_P = s->p.coulomb_type == RBMD_COULOMB_RBE ? s->S / (double)s->P : 1.0;
     k.f32 = s->v.mixed;
     int ewald = s->p.coulomb_type == RBMD_COULOMB_EWALD;
-    c->run("rho_partial", [&] { launch_rho_partial(s->v, k, c->stream); });
-    c->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, c->stream); });
+    c->run("rho_partial", [&] { launch_rho_partial(s->v, k, st); });
+    c->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, st); });
     int nblk = 0;
-    c->run("kforce", [&] {
-      launch_kforce(s->v, k, c->stream, ewald, s->p.alpha, thermo, 0, s->seg_rows(SEG_K), &nblk);
-    });
+    c->run("kforce", [&] { launch_kforce(s->v, k, st, ewald, s->p.alpha, thermo, 0, s->seg_rows(SEG_K), &nblk); });
     if (thermo) s->seg_used[SEG_K] = nblk;
   }
   if (has_bonded(s)) {
     int nblk = 0;
-    c->run("bonded", [&] { launch_bonded(s->v, s->iv, c->stream, thermo, s->seg_rows(SEG_BOND), &nblk); });
+    c->run("bonded", [&] { launch_bonded(s->v, s->iv, st, thermo, s->seg_rows(SEG_BOND), &nblk); });
     if (thermo) s->seg_used[SEG_BOND] = nblk;
   }
+}
+
+void compute_force(rbmd_sys* s, const double* kvec, int thermo) {
+  launch_real(s, thermo, 0);
+  launch_k_bonded(s, kvec, thermo, s->ctx->stream);
 }
 
 // Refill the k-vector rings of the systems whose ring half for `step` holds
@@ -636,7 +644,9 @@ rbmd_status rbmd_ctx_create(int32_t device, rbmd_ctx** out) {
   rbmd_ctx* c = new rbmd_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   if (e != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(RBMD_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
@@ -650,7 +660,9 @@ rbmd_status rbmd_ctx_destroy(rbmd_ctx* c) {
   c->flush();
   for (auto e : c->pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->side);
   (void)hipStreamDestroy(c->stream);
+  (void)hipStreamDestroy(c->side);
   delete c;
   return RBMD_OK;
 }
@@ -724,6 +736,12 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   rbmd_sys* s = new rbmd_sys();
   s->ctx = ctx;
   s->p = *p;
+  if (hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    delete s;
+    return fail(RBMD_ERR_HIP, "hipEventCreate");
+  }
   s->u = units_of(p->unit);
   int n = s->n = d->n_atoms;
   s->ntypes = d->n_types;
@@ -1057,6 +1075,8 @@ rbmd_status rbmd_sys_destroy(rbmd_sys* s) {
     if (s->stage[h]) (void)hipHostFree(s->stage[h]);
     if (s->stage_ev[h]) (void)hipEventDestroy(s->stage_ev[h]);
   }
+  (void)hipEventDestroy(s->ev_fork);
+  (void)hipEventDestroy(s->ev_join);
   delete s;
   return RBMD_OK;
 }
@@ -1498,8 +1518,23 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring) {
     s->seg_used[SEG_SHAKE] = (s->nmol + kBlock - 1) / kBlock;
     if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 1);
   }
+  // the positions are final for this step: the k-space and bonded forces
+  // branch off to the side stream while the list is built and the pair
+  // forces computed (kernel profiling keeps one stream, for its event timings)
+  const bool fork = !c->prof && (has_k(s) || has_bonded(s));
+  if (fork) {
+    (void)hipEventRecord(s->ev_fork, st);
+    (void)hipStreamWaitEvent(c->side, s->ev_fork, 0);
+    launch_k_bonded(s, kv, 1, c->side);
+    (void)hipEventRecord(s->ev_join, c->side);
+  }
   gather_and_list(s);
-  compute_force(s, kv, 1);
+  if (fork) {
+    launch_real(s, 1, 0);
+    (void)hipStreamWaitEvent(st, s->ev_join, 0);
+  } else {
+    compute_force(s, kv, 1);
+  }
   c->run("second_half", [&] {
     launch_second_half(s->v, s->iv, st, 1, has_k(s), has_bonded(s), s->seg_rows(SEG_KE), &s->seg_used[SEG_KE]);
   });
