@@ -173,7 +173,8 @@ struct rbmd_sys {
   double S = 0;
   double* kcur = nullptr;  // [3P] current k vectors
   double* kgraph = nullptr;  // [3P] k vectors of the running step inside a step graph
-  hipGraphExec_t graph = nullptr;  // one captured MD step (rbmd_md_run_graph)
+  // captured MD steps (rbmd_md_run_graph): [0] without, [1] with the thermo sums
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graph_ok = false;           // cleared whenever captured pointers or sizes change
   double* kring = nullptr; // [2R][3P]
   double* stage[2] = {nullptr, nullptr};
@@ -1069,7 +1070,8 @@ rbmd_status rbmd_sys_destroy(rbmd_sys* s) {
   if (!s) return RBMD_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
-  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  for (auto g : s->graph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (void* q : s->allocs) (void)hipFree(q);
   for (int h = 0; h < 2; ++h) {
     if (s->stage[h]) (void)hipHostFree(s->stage[h]);
@@ -1501,7 +1503,11 @@ static bool kring_on(const rbmd_sys* s) {
 // Enqueue one MD step (ExecutionNVT::PreSolve + Solve, ExecutionNVT.cpp:76-135)
 // with the k vectors at kv.  from_ring: inside a step graph; the step's k
 // vectors are copied from the ring slot the device step counter names.
-static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring) {
+// thermo: the step also forms the virial and energy sums behind pressure and
+// the reported energies; without them it computes the same forces and the
+// kinetic energy the thermostat needs, and the thermo state it leaves holds
+// no valid pressure or energies (the last step of every call has thermo on).
+static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool thermo) {
   rbmd_ctx* c = s->ctx;
   hipStream_t st = c->stream;
   const int tc = s->p.temp_ctrl;
@@ -1525,21 +1531,21 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring) {
   if (fork) {
     (void)hipEventRecord(s->ev_fork, st);
     (void)hipStreamWaitEvent(c->side, s->ev_fork, 0);
-    launch_k_bonded(s, kv, 1, c->side);
+    launch_k_bonded(s, kv, thermo, c->side);
     (void)hipEventRecord(s->ev_join, c->side);
   }
   gather_and_list(s);
   if (fork) {
-    launch_real(s, 1, 0);
+    launch_real(s, thermo, 0);
     (void)hipStreamWaitEvent(st, s->ev_join, 0);
   } else {
-    compute_force(s, kv, 1);
+    compute_force(s, kv, thermo);
   }
   c->run("second_half", [&] {
     launch_second_half(s->v, s->iv, st, 1, has_k(s), has_bonded(s), s->seg_rows(SEG_KE), &s->seg_used[SEG_KE]);
   });
   if (nh) {
-    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 2, st); });
+    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 2, st, thermo ? 0 : 1); });
     if (tc == RBMD_TEMP_NOSE_HOOVER && !(shake && s->nfree == 0))
       c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
     if (shake) {
@@ -1552,42 +1558,46 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring) {
       if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 0);
       c->run("ke_rows", [&] { launch_ke_rows(s->v, s->iv, st, s->seg_rows(SEG_KE)); });
     }
-    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 4, st); });
+    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 4, st, thermo ? 0 : 1); });
     c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
   }
 }
 
-static rbmd_status md_one_step(rbmd_sys* s) {
+static rbmd_status md_one_step(rbmd_sys* s, bool thermo) {
   const int64_t step = s->step + 1;
   const double* kv = nullptr;
   RC_TRY(kvec_for_step(s, step, &kv));
-  enqueue_step(s, kv, false);
+  enqueue_step(s, kv, false, thermo);
   s->step = step;
   return RBMD_OK;
 }
 
-// Capture one step into a graph.  The captured launches carry this system's
-// buffers and sizes, so the graph is rebuilt after anything changes them.
+// Capture one step into a graph, without and with the thermo sums.  The
+// captured launches carry this system's buffers and sizes, so the graphs are
+// rebuilt after anything changes them.
 static rbmd_status capture_step(rbmd_sys* s) {
   rbmd_ctx* c = s->ctx;
   hipStream_t st = c->stream;
-  if (s->graph) {
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-  }
+  for (auto& g : s->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   const bool ring = kring_on(s);
   if (ring && !s->kgraph) RC_TRY(s->dalloc(&s->kgraph, 3 * (size_t)s->P));
-  const bool prof = c->prof;
-  c->prof = false;
-  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-  enqueue_step(s, ring ? s->kgraph : s->kcur, ring);
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(st, &g);
-  c->prof = prof;
-  if (e != hipSuccess) return fail(RBMD_ERR_HIP, "step capture: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess) return fail(RBMD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  for (int thermo = 0; thermo < 2; ++thermo) {
+    const bool prof = c->prof;
+    c->prof = false;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    enqueue_step(s, ring ? s->kgraph : s->kcur, ring, thermo != 0);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    c->prof = prof;
+    if (e != hipSuccess) return fail(RBMD_ERR_HIP, "step capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&s->graph[thermo], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(RBMD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
   s->graph_ok = true;
   return RBMD_OK;
 }
@@ -1596,7 +1606,7 @@ rbmd_status rbmd_md_step(rbmd_sys* s, int64_t n_steps) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
   if (!s->initialized) return fail(RBMD_ERR_STATE, "rbmd_md_init has not run");
   for (int64_t k = 0; k < n_steps; ++k) {
-    RC_TRY(md_one_step(s));
+    RC_TRY(md_one_step(s, k == n_steps - 1));
     if (s->p.energy_every > 0 && has_k(s) && s->step % s->p.energy_every == 0) {
       RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
       s->energy_step = s->step;
@@ -1622,7 +1632,7 @@ static rbmd_status enqueue_graph_steps(rbmd_sys* s, int64_t n_steps) {
       const double* unused = nullptr;
       RC_TRY(kvec_for_step(s, step, &unused));  // fills the ring half this step reads
     }
-    HIP_TRY(hipGraphLaunch(s->graph, st));
+    HIP_TRY(hipGraphLaunch(s->graph[k == n_steps - 1 ? 1 : 0], st));
     s->step = step;
     if (s->p.energy_every > 0 && has_k(s) && s->step % s->p.energy_every == 0) {
       RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
@@ -1732,7 +1742,7 @@ struct rbmd_pack {
   PackLaunch L[PK_COUNT];
   int nblk[PK_COUNT] = {};
   size_t lds_first = 0, lds_scatter = 0;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // captured packed step without / with thermo sums
   std::vector<void*> allocs;
 };
 
@@ -1756,16 +1766,19 @@ static RowsDesc pack_rows_desc(rbmd_sys* s, bool has_kk, bool has_bb, bool shk) 
   return d;
 }
 
-static void pack_enqueue_step(rbmd_pack* p) {
+// thermo: as for enqueue_step (the last step of every call has it on)
+static void pack_enqueue_step(rbmd_pack* p, bool thermo) {
   rbmd_ctx* c = p->ctx;
   hipStream_t st = c->stream;
+  PackSig sig = p->sig;
+  sig.thermo = thermo ? 1 : 0;
   auto integ = [&](int k, int arg, size_t lds) {
-    c->run(kPackName[k], [&] { launch_pack_integrate(k, p->L[k], p->nblk[k], p->sig, arg, lds, nullptr, st); });
+    c->run(kPackName[k], [&] { launch_pack_integrate(k, p->L[k], p->nblk[k], sig, arg, lds, nullptr, st); });
   };
   auto cells = [&](int k, size_t lds) {
     c->run(kPackName[k], [&] { launch_pack_cells(k, p->L[k], p->nblk[k], lds, st); });
   };
-  auto ksp = [&](int k) { c->run(kPackName[k], [&] { launch_pack_kspace(k, p->L[k], p->nblk[k], p->sig, st); }); };
+  auto ksp = [&](int k) { c->run(kPackName[k], [&] { launch_pack_kspace(k, p->L[k], p->nblk[k], sig, st); }); };
   integ(PK_NHC, p->nh ? 1 : 5, 0);
   if (p->ring) ksp(PK_KSLOT);
   integ(PK_FIRST_HALF, p->tc == RBMD_TEMP_NOSE_HOOVER ? 1 : 0, p->lds_first);
@@ -1775,7 +1788,7 @@ static void pack_enqueue_step(rbmd_pack* p) {
   if (p->shake) integ(PK_SHAKE, 0, 0);
   cells(PK_GATHER, 0);
   cells(PK_NBR, 0);
-  c->run(kPackName[PK_PAIR], [&] { launch_pack_pair(p->L[PK_PAIR], p->nblk[PK_PAIR], p->sig, st); });
+  c->run(kPackName[PK_PAIR], [&] { launch_pack_pair(p->L[PK_PAIR], p->nblk[PK_PAIR], sig, st); });
   if (p->has_k) {
     ksp(PK_RHO);
     ksp(PK_RHO_RED);
@@ -1842,6 +1855,7 @@ rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n,
   p->sig.has_b = p->has_b;
   p->sig.k_lanes_lg = s0->v.k_lanes_lg;
   p->sig.rho_kb = s0->v.rho_kb;
+  p->sig.thermo = 1;
   auto dalloc = [&](void** q, size_t bytes) -> rbmd_status {
     hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 4));
     if (e != hipSuccess) return fail(RBMD_ERR_OOM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
@@ -1931,7 +1945,8 @@ rbmd_status rbmd_pack_destroy(rbmd_pack* p) {
   if (!p) return RBMD_OK;
   (void)hipSetDevice(p->ctx->device);
   (void)hipStreamSynchronize(p->ctx->stream);
-  if (p->graph) (void)hipGraphExecDestroy(p->graph);
+  for (auto g : p->graph)
+    if (g) (void)hipGraphExecDestroy(g);
   return pack_fail_free(p, RBMD_OK);
 }
 
@@ -1939,16 +1954,17 @@ rbmd_status rbmd_pack_destroy(rbmd_pack* p) {
 static rbmd_status pack_steps(rbmd_pack* p, int64_t n_steps, bool graph) {
   if (!p) return fail(RBMD_ERR_ARG, "null pack");
   hipStream_t st = p->ctx->stream;
-  if (graph && !p->graph) {
+  for (int thermo = 0; graph && thermo < 2; ++thermo) {
+    if (p->graph[thermo]) continue;
     const bool prof = p->ctx->prof;
     p->ctx->prof = false;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-    pack_enqueue_step(p);
+    pack_enqueue_step(p, thermo != 0);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
     p->ctx->prof = prof;
     if (e != hipSuccess) return fail(RBMD_ERR_HIP, "pack step capture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&p->graph[thermo], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(RBMD_ERR_HIP, "pack graph instantiate: %s", hipGetErrorString(e));
   }
@@ -1956,10 +1972,11 @@ static rbmd_status pack_steps(rbmd_pack* p, int64_t n_steps, bool graph) {
   for (int64_t k = 0; k < n_steps; ++k) {
     for (size_t q = 0; q < p->sys.size(); ++q) next[q] = p->sys[q]->step + 1;
     RC_TRY(ring_fill(p->sys.data(), next.data(), (int)p->sys.size()));
+    const bool thermo = k == n_steps - 1;
     if (graph)
-      HIP_TRY(hipGraphLaunch(p->graph, st));
+      HIP_TRY(hipGraphLaunch(p->graph[thermo ? 1 : 0], st));
     else
-      pack_enqueue_step(p);
+      pack_enqueue_step(p, thermo);
     for (rbmd_sys* s : p->sys) s->step += 1;
   }
   return check_launch("pack steps");

@@ -215,7 +215,9 @@ struct NhcParams {
 };
 // phase 1: step++ and NHC half step (InitialIntegrate); 2: ComputeTempe + pressure + NHC
 // (FinalIntegrate); 3: SetUp; 4: Berendsen/rescale end of step; 5: step++ only
-void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st);
+// ke_only: reduce the kinetic-energy segment only (steps without thermo sums)
+void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st,
+                     int ke_only = 0);
 
 // ---------------------------------------------------------------- packs
 // A pack runs the MD steps of many independent systems with one launch per
@@ -274,6 +276,7 @@ int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel);
 
 struct PackSig {  // launch-wide choices, equal for every system of a pack
   int pair_mode, pair_sp, f32, has_k, ewald, has_b, k_lanes_lg, rho_kb;
+  int thermo;  // this step computes the virial and energy sums (else forces and KE only)
 };
 
 // Packed launches of one kernel (nblocks = total over the pack; lds = the

@@ -27,7 +27,8 @@
 // The three column accumulators of each atom are added in wavefront order and
 // stored as the slab-a partial (fn[a + 1][i]); consumers add the three slabs
 // in order.  Workgroups past the work items take the misfit atoms (current
-// cell != binned cell, e.g. after SHAKE): one atom at a time, lanes =
+// cell != binned cell, e.g. after SHAKE): each a fixed range of sorted
+// positions, its misfit atoms one at a time in increasing position, lanes =
 // candidates of all 9 columns, and the whole force goes to slab 0.
 #include "dev.hpp"
 #include "reduce.hpp"
@@ -98,13 +99,23 @@ __device__ __forceinline__ void pair_vector(const SysView& s, double& rx, double
 template <int MODE, bool THERMO, bool PERATOM, int SP>
 __device__ void misfit_force(const SysView& s, const double2* ctab, const double* teps, const double* tsig, int wave,
                              int lane, double* buf, ThermoAcc& tacc) {
+#pragma clang fp contract(off)  // fused multiply-adds are written out (see pair_term)
   constexpr bool EN = THERMO || PERATOM;
   constexpr int NV = PERATOM ? 11 : 3;
-  const int count = s.misfit_count[0];
+  if (s.misfit_count[0] == 0) return;
   const int b = wave - 1;
   const int nt = s.ntypes;
-  for (int m = blockIdx.x - kSlabWaves * s.n_items; m < count; m += kMisfitBlocks) {
-    const int i = __builtin_amdgcn_readfirstlane(s.misfit_list[m]);
+  // this workgroup's range of sorted positions; its misfit atoms are taken in
+  // increasing position, so the thermo sums it accumulates have a fixed order
+  // (the misfit list itself is appended in atomic order)
+  const int per = (s.n + kMisfitBlocks - 1) / kMisfitBlocks;
+  const int p0 = ((int)blockIdx.x - kSlabWaves * s.n_items) * per;
+  const int p1 = p0 + per < s.n ? p0 + per : s.n;
+  for (int base = p0; base < p1; base += kWave) {
+   unsigned long long flagged = __ballot(base + lane < p1 && s.misfit[base + lane] != 0);
+   while (flagged) {
+    const int i = base + __builtin_ctzll(flagged);
+    flagged &= flagged - 1ull;
     const double4 pi = s.spos[i];
     const int ti = s.stype[i];
     int spj[SP > 0 ? SP : 1];
@@ -130,16 +141,17 @@ __device__ void misfit_force(const SysView& s, const double2* ctab, const double
         if (!(s.cut2 - dd > 0.0001 && dd > 0.0001)) continue;
         const int j = entry >> kShiftBits;
         pair_vector(s, rx, ry, rz);
-        const double d2 = rx * rx + ry * ry + rz * rz;
+        const double d2 = fma(rz, rz, fma(ry, ry, rx * rx));
         const int tix = ti * nt + s.stype[j];
         const double w = MODE == 1 ? special_weight<SP>(spj, spw, j) : 1.0;
         double elj, ec;
         const double sc = pair_term<MODE, EN>(s, ctab, d2, pi.w * pj.w, teps[tix], tsig[tix], w, elj, ec);
-        const double px = sc * rx, py = sc * ry, pz = sc * rz;
-        v[0] += px;
-        v[1] += py;
-        v[2] += pz;
+        // one fused rounding per force term
+        v[0] = fma(sc, rx, v[0]);
+        v[1] = fma(sc, ry, v[1]);
+        v[2] = fma(sc, rz, v[2]);
         if (EN) {
+          const double px = sc * rx, py = sc * ry, pz = sc * rz;
           double vv[6] = {0, 0, 0, 0, 0, 0};
           vir_add(vv, rx, ry, rz, px, py, pz, MODE == 1 ? -0.5 : 0.5);
           for (int k = 0; k < 6; ++k) tacc.vir[k] += vv[k];
@@ -178,6 +190,7 @@ __device__ void misfit_force(const SysView& s, const double2* ctab, const double
           s.e_coul[sl * n + i] = sl == 0 ? v[10] : 0.0;
         }
     }
+   }
   }
 }
 
@@ -185,6 +198,7 @@ __device__ void misfit_force(const SysView& s, const double2* ctab, const double
 
 template <int MODE, bool THERMO, bool PERATOM, int SP>
 __global__ __launch_bounds__(kSlabThreads) void k_force_cells(SysView s, double* rows) {
+#pragma clang fp contract(off)  // fused multiply-adds are written out (see pair_term)
   constexpr bool EN = THERMO || PERATOM;
   constexpr int SPD = SP > 0 ? SP : 1;
   constexpr int NACC = PERATOM ? 11 : 3;  // per-atom accumulators: force (+ virial, energies)
@@ -289,7 +303,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_force_cells(SysView s, double*
         const int tix = ityp[ii] * nt + stage_t[wave][u];
         double rx = q.x - p.x, ry = q.y - p.y, rz = q.z - p.z;
         pair_vector(s, rx, ry, rz);
-        const double d2 = rx * rx + ry * ry + rz * rz;
+        const double d2 = fma(rz, rz, fma(ry, ry, rx * rx));
         const double w = MODE == 1 ? special_weight<SP>(isp_j[ii], isp_w[ii], j) : 1.0;
         double elj, ec;
         const double sc = pair_term<MODE, EN>(s, ctab, d2, p.w * q.w, teps[tix], tsig[tix], w, elj, ec);

@@ -654,7 +654,7 @@ __device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, 
 // thermostat state.
 // nseg: thermo row segments to reduce (rd.nseg, or 0 for no reduction)
 __device__ __forceinline__ void nhc_body(NhcState* state, const NhcParams& p, const RowsDesc& rd, const int nseg,
-                                         int phase, const int bid) {
+                                         int phase, const int ke_only, const int bid) {
   // All producer segments at once: segment g owns threads [base, base +
   // groups * width); thread (group, k) strides rows group, group + groups, ...
   // of column k (consecutive threads read consecutive doubles), then thread k
@@ -675,8 +675,9 @@ __device__ __forceinline__ void nhc_body(NhcState* state, const NhcParams& p, co
   if (mine >= 0) {
     const RowSeg sg = rd.seg[mine];
     const int k = (tid - mbase) % sg.width, grp = (tid - mbase) / sg.width;
+    const int nrows = (ke_only && sg.slot0 != S_KE) ? 0 : sg.nrows;  // skipped sums read as zero
     double acc = 0.0;
-    for (int r = grp; r < sg.nrows; r += sg.groups) acc += sg.rows[(size_t)r * sg.width + k];
+    for (int r = grp; r < nrows; r += sg.groups) acc += sg.rows[(size_t)r * sg.width + k];
     part[tid] = acc;
   }
   __syncthreads();
@@ -691,21 +692,22 @@ __device__ __forceinline__ void nhc_body(NhcState* state, const NhcParams& p, co
   nhc_update(state, p, sums, phase);
 }
 
-__global__ __launch_bounds__(1024) void k_nhc(NhcState* state, NhcParams p, RowsDesc rd, int phase) {
-  nhc_body(state, p, rd, rd.nseg, phase, (int)blockIdx.x);
+__global__ __launch_bounds__(1024) void k_nhc(NhcState* state, NhcParams p, RowsDesc rd, int phase, int ke_only) {
+  nhc_body(state, p, rd, rd.nseg, phase, ke_only, (int)blockIdx.x);
 }
 
-void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st) {
-  hipLaunchKernelGGL(k_nhc, dim3(1), dim3(1024), 0, st, st_dev, p, rd, phase);
+void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st,
+                     int ke_only) {
+  hipLaunchKernelGGL(k_nhc, dim3(1), dim3(1024), 0, st, st_dev, p, rd, phase, ke_only);
 }
 
 // ------------------------------------------------------------------ packs
 // The same bodies as the kernels above, one launch over every system of a
 // pack; each block takes its system's arguments from the pack table.
 
-__global__ __launch_bounds__(1024) void kp_nhc(PACK_PARAMS, int phase, int reduce) {
+__global__ __launch_bounds__(1024) void kp_nhc(PACK_PARAMS, int phase, int reduce, int ke_only) {
   PACK_ENTRY(e, bid)
-  nhc_body(e.nhc, e.np, e.rd, reduce ? e.rd.nseg : 0, phase, bid);
+  nhc_body(e.nhc, e.np, e.rd, reduce ? e.rd.nseg : 0, phase, ke_only, bid);
 }
 
 __global__ __launch_bounds__(kBlock) void kp_first_half(PACK_PARAMS, int use_scale) {
@@ -729,9 +731,10 @@ __global__ __launch_bounds__(kBlock) void kp_shake(PACK_PARAMS) {
   shake_body(e.v, e.iv, e.rows_shake, e.shake_iter, bid);
 }
 
+template <bool THERMO>
 __global__ __launch_bounds__(kBlock) void kp_bonded(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
-  bonded_body<true>(e.v, e.iv, e.rows_bond, bid);
+  bonded_body<THERMO>(e.v, e.iv, e.rows_bond, bid);
 }
 
 __global__ __launch_bounds__(kBlock) void kp_second_half(PACK_PARAMS, int kick, int has_k, int has_b) {
@@ -755,7 +758,7 @@ void launch_pack_integrate(int kernel, const PackLaunch& L, int nblocks, const P
   const dim3 grid(nblocks);
   switch (kernel) {
     case PK_NHC:
-      hipLaunchKernelGGL(kp_nhc, grid, dim3(1024), 0, st, PACK_ARGS(L), arg & 7, (arg >> 3) & 1);
+      hipLaunchKernelGGL(kp_nhc, grid, dim3(1024), 0, st, PACK_ARGS(L), arg & 7, (arg >> 3) & 1, g.thermo ? 0 : 1);
       break;
     case PK_FIRST_HALF:
       hipLaunchKernelGGL(kp_first_half, grid, dim3(kBlock), lds, st, PACK_ARGS(L), arg);
@@ -767,7 +770,10 @@ void launch_pack_integrate(int kernel, const PackLaunch& L, int nblocks, const P
       hipLaunchKernelGGL(kp_shake, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
       break;
     case PK_BONDED:
-      hipLaunchKernelGGL(kp_bonded, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      if (g.thermo)
+        hipLaunchKernelGGL(kp_bonded<true>, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      else
+        hipLaunchKernelGGL(kp_bonded<false>, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
       break;
     case PK_SECOND:
       hipLaunchKernelGGL(kp_second_half, grid, dim3(kBlock), 0, st, PACK_ARGS(L), arg, g.has_k, g.has_b);

@@ -31,7 +31,7 @@ constexpr int kRhoAtomsPerThread = 4;
 template <bool F32>
 __device__ __forceinline__ void phase_sincos(double mx, double my, double mz, double sx, double sy, double sz,
                                              double* sn, double* cs) {
-  const double t = mx * sx + my * sy + mz * sz;
+  const double t = fma(mz, sz, fma(my, sy, mx * sx));
   if constexpr (F32) {
     float s32, c32;
     sincos_turns32(t, &s32, &c32);
@@ -49,6 +49,7 @@ __device__ __forceinline__ double k_to_m(double k, double box) { return rint(k *
 // bk: block of KB k vectors, ba: block of kBlock * kRhoAtomsPerThread atoms
 template <bool F32, int KB>
 __device__ __forceinline__ void rho_partial_body(const SysView& s, const KView& kv, const int bk, const int ba) {
+#pragma clang fp contract(off)  // fused multiply-adds are written out: the same rounding in every kernel
   constexpr int kRhoK = KB;
   __shared__ double lds[kBlock / kWave][2 * kRhoK];
   const int l0 = bk * kRhoK;
@@ -75,8 +76,8 @@ __device__ __forceinline__ void rho_partial_body(const SysView& s, const KView& 
       for (int t = 0; t < kRhoK; ++t) {
         double sn, cs;
         phase_sincos<F32>(mx[t], my[t], mz[t], sx, sy, sz, &sn, &cs);
-        re[t] += q * cs;
-        im[t] += q * sn;
+        re[t] = fma(q, cs, re[t]);
+        im[t] = fma(q, sn, im[t]);
       }
     }
   }
@@ -171,6 +172,7 @@ constexpr int kForceTile = 256;
 // lane `sub` takes k-vectors sub, sub + 2^LG, ...
 template <bool THERMO, bool PERATOM, bool F32, int LG>
 __device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, int ewald, double alpha, double* rows, const int bid) {
+#pragma clang fp contract(off)  // fused multiply-adds are written out: the same forces with or without the virial sums
   constexpr int kKLg = LG;
   constexpr int L = 1 << kKLg;
   __shared__ double tk[kForceTile][4];
@@ -216,10 +218,10 @@ __device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, i
       double kx = tk[u][0], ky = tk[u][1], kz = tk[u][2];
       double sn, cs;
       phase_sincos<F32>(tm[u][0], tm[u][1], tm[u][2], sx, sy, sz, &sn, &cs);
-      double fs = (-4.0 * kPi * q) * tk[u][3] * (cs * tr[u][1] - sn * tr[u][0]);
-      fx += fs * kx;
-      fy += fs * ky;
-      fz += fs * kz;
+      const double fs = (-4.0 * kPi * q) * tk[u][3] * fma(cs, tr[u][1], -(sn * tr[u][0]));
+      fx = fma(fs, kx, fx);
+      fy = fma(fs, ky, fy);
+      fz = fma(fs, kz, fz);
       if (THERMO || PERATOM) {
         v[0] += (x * kx + x * kx) * fs;
         v[1] += (y * ky + y * ky) * fs;
@@ -310,10 +312,10 @@ __global__ __launch_bounds__(kBlock) void kp_rho_reduce(PACK_PARAMS) {
   rho_reduce_body(e.kv, bid);
 }
 
-template <bool F32, int LG>
+template <bool F32, int LG, bool THERMO>
 __global__ __launch_bounds__(kBlock) void kp_kforce(PACK_PARAMS, int ewald) {
   PACK_ENTRY(e, bid)
-  kforce_body<true, false, F32, LG>(e.v, e.kv, ewald, e.v.alpha, e.rows_k, bid);
+  kforce_body<THERMO, false, F32, LG>(e.v, e.kv, ewald, e.v.alpha, e.rows_k, bid);
 }
 
 void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const PackSig& g, hipStream_t st) {
@@ -332,8 +334,13 @@ void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const Pack
       hipLaunchKernelGGL(kp_rho_reduce, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
       break;
     case PK_KFORCE:
-#define KP_KF_LAUNCH(F32_, LG_) \
-  hipLaunchKernelGGL((kp_kforce<F32_, LG_>), grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald)
+#define KP_KF_LAUNCH(F32_, LG_)                                                                      \
+  do {                                                                                               \
+    if (g.thermo)                                                                                    \
+      hipLaunchKernelGGL((kp_kforce<F32_, LG_, true>), grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald);  \
+    else                                                                                             \
+      hipLaunchKernelGGL((kp_kforce<F32_, LG_, false>), grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald); \
+  } while (0)
       if (g.k_lanes_lg == 1) {
         if (g.f32) KP_KF_LAUNCH(true, 1); else KP_KF_LAUNCH(false, 1);
       } else if (g.k_lanes_lg == 3) {

@@ -68,6 +68,7 @@ __device__ __forceinline__ int cursor_take(ListCursor& c, const int (&cnt)[kMaxS
 // carry no more rounding than the FP64 path's).
 template <int MODE, bool THERMO, bool PERATOM, int SP, bool LTAB, bool F32>
 __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows, const int bid) {
+#pragma clang fp contract(off)  // fused multiply-adds are written out (see pair_term)
   constexpr bool EN = THERMO || PERATOM;
   constexpr int SPD = SP > 0 ? SP : 1;
   using TT = typename std::conditional<F32, float, double>::type;
@@ -153,7 +154,7 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
         double ry = (p.y - oy) - pi.y;
         double rz = (p.z - oz) - pi.z;
         if (s.use_min_image) min_image(s, rx, ry, rz);
-        const double d2 = rx * rx + ry * ry + rz * rz;
+        const double d2 = fma(rz, rz, fma(ry, ry, rx * rx));
         const int tix = ti * nt + tj;
         const TT epsij = LTAB ? teps[tix] : (TT)s.eps_ij[tix];
         const TT sig6 = LTAB ? tsig[tix] : (TT)s.sig6_ij[tix];
@@ -177,11 +178,12 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
           sc = s.use_min_image ? pair_term<MODE, EN, true>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec)
                                : pair_term<MODE, EN, false>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec);
         }
-        const double px = sc * rx, py = sc * ry, pz = sc * rz;
-        fx += px;
-        fy += py;
-        fz += pz;
+        // one fused rounding per force term
+        fx = fma(sc, rx, fx);
+        fy = fma(sc, ry, fy);
+        fz = fma(sc, rz, fz);
         if (EN) {
+          const double px = sc * rx, py = sc * ry, pz = sc * rz;
           vir_add(vir, rx, ry, rz, px, py, pz, 1.0);  // the +-1/2 is applied once at the end
           pelj += elj;
           pec += ec;
@@ -418,32 +420,41 @@ void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, doub
 }
 
 // ------------------------------------------------------------------ packs
-// thermo on, per-atom outputs off, LDS type tables (packs need ntypes <= kMaxTypes)
+// per-atom outputs off, LDS type tables (packs need ntypes <= kMaxTypes);
+// thermo sums on the steps that report them
 
-template <int MODE, int SP, bool F32>
+template <int MODE, int SP, bool F32, bool THERMO>
 __global__ __launch_bounds__(kBlock) void kp_pair(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
-  pair_verlet_body<MODE, true, false, SP, true, F32>(e.v, e.rows_pair, bid);
+  pair_verlet_body<MODE, THERMO, false, SP, true, F32>(e.v, e.rows_pair, bid);
+}
+
+template <int MODE, int SP, bool F32>
+void launch_kp_pair_t(const PackLaunch& L, int nblocks, int thermo, hipStream_t st) {
+  if (thermo)
+    hipLaunchKernelGGL((kp_pair<MODE, SP, F32, true>), dim3(nblocks), dim3(kBlock), 0, st, PACK_ARGS(L));
+  else
+    hipLaunchKernelGGL((kp_pair<MODE, SP, F32, false>), dim3(nblocks), dim3(kBlock), 0, st, PACK_ARGS(L));
 }
 
 template <int MODE, int SP>
-void launch_kp_pair(const PackLaunch& L, int nblocks, int f32, hipStream_t st) {
+void launch_kp_pair(const PackLaunch& L, int nblocks, int f32, int thermo, hipStream_t st) {
   if (f32)
-    hipLaunchKernelGGL((kp_pair<MODE, SP, true>), dim3(nblocks), dim3(kBlock), 0, st, PACK_ARGS(L));
+    launch_kp_pair_t<MODE, SP, true>(L, nblocks, thermo, st);
   else
-    hipLaunchKernelGGL((kp_pair<MODE, SP, false>), dim3(nblocks), dim3(kBlock), 0, st, PACK_ARGS(L));
+    launch_kp_pair_t<MODE, SP, false>(L, nblocks, thermo, st);
 }
 
 void launch_pack_pair(const PackLaunch& L, int nblocks, const PackSig& g, hipStream_t st) {
   if (nblocks <= 0) return;
   if (g.pair_mode == 2)
-    launch_kp_pair<2, 0>(L, nblocks, g.f32, st);
+    launch_kp_pair<2, 0>(L, nblocks, g.f32, g.thermo, st);
   else if (g.pair_mode == 0)
-    launch_kp_pair<0, 0>(L, nblocks, g.f32, st);
+    launch_kp_pair<0, 0>(L, nblocks, g.f32, g.thermo, st);
   else if (g.pair_sp <= 4)
-    launch_kp_pair<1, 4>(L, nblocks, g.f32, st);
+    launch_kp_pair<1, 4>(L, nblocks, g.f32, g.thermo, st);
   else
-    launch_kp_pair<1, 16>(L, nblocks, g.f32, st);
+    launch_kp_pair<1, 16>(L, nblocks, g.f32, g.thermo, st);
 }
 
 }  // namespace rbmd

@@ -77,9 +77,16 @@ __device__ __forceinline__ double coul_poly(const double2* ctab, int k, double u
 // P and Q come from the LDS copy of the piecewise polynomial table.  CHECK:
 // test 1e-4 < d2 < rc^2 (off for pairs already accepted by the list predicate
 // with the same vector).
+//
+// The force functions compile with FMA contraction off and their fused
+// multiply-adds written out: the compiler would otherwise choose what to fuse
+// per kernel (by what else reads a product), and the kernels with and without
+// the energy and virial sums, alone and packed, would round the same force
+// differently.
 template <int MODE, bool EN, bool CHECK = true>
 __device__ __forceinline__ double pair_term(const SysView& s, const double2* ctab, double d2, double qq, double epsij,
                                             double sig6, double w, double& elj, double& ec) {
+#pragma clang fp contract(off)
   const bool in = !CHECK || (d2 < s.cut2 && d2 > 0.0001);
   double flj = 0, fc = 0;
   elj = 0;
@@ -88,7 +95,7 @@ __device__ __forceinline__ double pair_term(const SysView& s, const double2* cta
     const double rinv = rsqrt(d2);
     const double rinv2 = rinv * rinv;
     const double sd6 = sig6 * (rinv2 * rinv2 * rinv2);
-    flj = -24.0 * epsij * ((2.0 * sd6 - 1.0) * sd6) * rinv2;
+    flj = -24.0 * epsij * (fma(2.0, sd6, -1.0) * sd6) * rinv2;
     if (EN) elj = 0.5 * (4.0 * epsij * (sd6 - 1.0) * sd6);
     if (MODE != 2) {
       const double fk = (d2 * rinv) * s.ctab_inv_h;
@@ -138,6 +145,7 @@ __device__ __forceinline__ float coul_poly32(const float2* ctab, int k, float u)
 template <int MODE, bool EN, bool CHECK = true>
 __device__ __forceinline__ float pair_term32(const SysView& s, const float2* ctab, double d2d, float qq, float epsij,
                                              float sig6, float w, float& elj, float& ec) {
+#pragma clang fp contract(off)
   const bool in = !CHECK || (d2d < s.cut2 && d2d > 0.0001);
   const float d2 = (float)d2d;
   float flj = 0.f, fc = 0.f;
@@ -147,7 +155,7 @@ __device__ __forceinline__ float pair_term32(const SysView& s, const float2* cta
     const float rinv = __builtin_amdgcn_rsqf(d2);
     const float rinv2 = rinv * rinv;
     const float sd6 = sig6 * (rinv2 * rinv2 * rinv2);
-    flj = -24.0f * epsij * ((2.0f * sd6 - 1.0f) * sd6) * rinv2;
+    flj = -24.0f * epsij * (fmaf(2.0f, sd6, -1.0f) * sd6) * rinv2;
     if (EN) elj = 0.5f * (4.0f * epsij * (sd6 - 1.0f) * sd6);
     if (MODE != 2) {
       const float fk = (d2 * rinv) * (float)s.ctab_inv_h;
