@@ -24,6 +24,15 @@ constexpr int kMaxTypes = 16;   // type-pair tables held in LDS up to this many 
 
 // Neighbour-list slot layout: one row of `cap` slots per atom (sorted order),
 // so a row's consecutive entries are consecutive words.
+// |r|^2 of a pair vector as the reference's host code and the CPU oracle
+// round it: x*x + y*y, then + z*z, every operation rounded (no fused
+// multiply-add), so the cutoff predicate lists exactly the same pairs on
+// every path, also for pairs that sit at the cutoff to the last bit.
+__device__ __forceinline__ double dist2(double x, double y, double z) {
+#pragma clang fp contract(off)
+  return x * x + y * y + z * z;
+}
+
 __host__ __device__ __forceinline__ size_t nbr_index(int slot, int i, int cap) { return (size_t)i * cap + slot; }
 
 // Everything a kernel needs to know about one system.  Passed by value.

@@ -11,10 +11,11 @@
 // ComputeRBLNeighboursOnceWorklet (:906-980): the (2nc+1)^3 stencil around
 // InWhichCell(p_i) in x-outer / z-inner order, cells in sorted order, accept
 // when rc^2 - d^2 > 1e-4 and d^2 > 1e-4 (RBL: core d < rs, every pice_num-th
-// shell atom).  One wavefront owns 64 i-atoms of a cell; the j loop is
-// wave-uniform, so j positions come through the scalar cache (s_load) and are
-// broadcast to all lanes.  Lists are column-major [slot][n] so that the pair
-// kernel's per-lane reads of slot k are coalesced.
+// shell atom), with d^2 rounded as the oracle rounds it (dist2).  One
+// wavefront owns 64 i-atoms of a cell and one stencil column (VERLET) or
+// the whole stencil (RBL); candidates are loaded coalesced and broadcast
+// through LDS.  Lists are row-major (one row of segments per atom), so the
+// pair kernel's lanes of one atom read consecutive entries.
 #include "dev.hpp"
 #include "pack.hpp"
 #include "reduce.hpp"
@@ -274,7 +275,7 @@ __device__ void misfit_block(const SysView& s, int wave, int lane, int bid) {
         double4 pj;
         COLUMN_CANDIDATE(t, pj, entry);
         double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-        double d2 = rx * rx + ry * ry + rz * rz;
+        double d2 = dist2(rx, ry, rz);
         ok = s.cut2 - d2 > 0.0001 && d2 > 0.0001;
       }
       const unsigned long long bal = __ballot(ok);
@@ -382,7 +383,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
         const double rx = q[r].x - pi.x;
         const double ry = q[r].y - pi.y;
         const double rz = q[r].z - pi.z;
-        const double d2 = rx * rx + ry * ry + rz * rz;
+        const double d2 = dist2(rx, ry, rz);
         ok[r] = active & (cut2 - d2 > 0.0001) & (d2 > 0.0001);
       }
 #pragma unroll
@@ -398,7 +399,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
       const double rx = q.x - pi.x;
       const double ry = q.y - pi.y;
       const double rz = q.z - pi.z;
-      const double d2 = rx * rx + ry * ry + rz * rz;
+      const double d2 = dist2(rx, ry, rz);
       if (active & (cut2 - d2 > 0.0001) & (d2 > 0.0001)) {
         if (cnt < seg_cap) row[cnt] = __double2loint(q.w);
         ++cnt;
@@ -486,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_build_rbl(SysView s) {
           double rx = (pj.x - t.ox) - pi.x;
           double ry = (pj.y - t.oy) - pi.y;
           double rz = (pj.z - t.oz) - pi.z;
-          double d2 = rx * rx + ry * ry + rz * rz;
+          double d2 = dist2(rx, ry, rz);
           if (active) rbl_consider(s, L, i, (j << kShiftBits) | code, d2);
         }
       }
@@ -527,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_misfit_rbl(SysView s) {
               double rx = (pj.x - t.ox) - pi.x;
               double ry = (pj.y - t.oy) - pi.y;
               double rz = (pj.z - t.oz) - pi.z;
-              double d2 = rx * rx + ry * ry + rz * rz;
+              double d2 = dist2(rx, ry, rz);
               is_shell = s.cut2 - d2 > 0.0001 && d2 - s.rs2 > 0.0001;
               is_core = !is_shell && s.rs2 - d2 > 0.0001 && d2 > 0.0001;
             }

@@ -137,11 +137,11 @@ __device__ void misfit_force(const SysView& s, const double2* ctab, const double
         int entry;
         COLUMN_CANDIDATE(t, pj, entry);
         double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-        const double dd = rx * rx + ry * ry + rz * rz;
+        const double dd = dist2(rx, ry, rz);
         if (!(s.cut2 - dd > 0.0001 && dd > 0.0001)) continue;
         const int j = entry >> kShiftBits;
         pair_vector(s, rx, ry, rz);
-        const double d2 = fma(rz, rz, fma(ry, ry, rx * rx));
+        const double d2 = dist2(rx, ry, rz);
         const int tix = ti * nt + s.stype[j];
         const double w = MODE == 1 ? special_weight<SP>(spj, spw, j) : 1.0;
         double elj, ec;
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_force_cells(SysView s, double*
         const int tix = ityp[ii] * nt + stage_t[wave][u];
         double rx = q.x - p.x, ry = q.y - p.y, rz = q.z - p.z;
         pair_vector(s, rx, ry, rz);
-        const double d2 = fma(rz, rz, fma(ry, ry, rx * rx));
+        const double d2 = dist2(rx, ry, rz);
         const double w = MODE == 1 ? special_weight<SP>(isp_j[ii], isp_w[ii], j) : 1.0;
         double elj, ec;
         const double sc = pair_term<MODE, EN>(s, ctab, d2, p.w * q.w, teps[tix], tsig[tix], w, elj, ec);
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_force_cells(SysView s, double*
           if (u < kc) {
             const double4 q = stage[wave][u];
             const double rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;
-            const double d2 = rx * rx + ry * ry + rz * rz;
+            const double d2 = dist2(rx, ry, rz);
             ok = active & (cut2 - d2 > 0.0001) & (d2 > 0.0001);
           }
           const unsigned long long acc = __ballot(ok);

@@ -154,7 +154,7 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
         double ry = (p.y - oy) - pi.y;
         double rz = (p.z - oz) - pi.z;
         if (s.use_min_image) min_image(s, rx, ry, rz);
-        const double d2 = fma(rz, rz, fma(ry, ry, rx * rx));
+        const double d2 = dist2(rx, ry, rz);
         const int tix = ti * nt + tj;
         const TT epsij = LTAB ? teps[tix] : (TT)s.eps_ij[tix];
         const TT sig6 = LTAB ? tsig[tix] : (TT)s.sig6_ij[tix];
@@ -263,6 +263,7 @@ void launch_pair_t(const SysView& s, hipStream_t st, int thermo, int peratom, do
 // reference's assignment of the running-sum force of the last pair (hazard Q5).
 template <bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void k_pair_rbl(SysView s) {
+#pragma clang fp contract(off)  // the pair vector and d2 rounded as the list build and the oracle round them
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= s.n) return;
   const double4 pi = s.spos[i];
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_rbl(SysView s) {
       double ry = (pj.y - oy) - pi.y;
       double rz = (pj.z - oz) - pi.z;
       min_image(s, rx, ry, rz);
-      const double d2 = rx * rx + ry * ry + rz * rz;
+      const double d2 = dist2(rx, ry, rz);
       const double dis = sqrt(d2);
       const int tj = s.stype[j];
       const double epsij = eps_row[tj];
