@@ -7,6 +7,9 @@
 namespace rbmd {
 
 constexpr int kMisfitBlocks = 64;  // workgroups that take the misfit atoms
+constexpr int kColWaves = 9;        // stencil columns of the VERLET list build
+constexpr int kNbrWaves = 3;        // columns (wavefronts) per list-build workgroup
+constexpr int kNbrParts = kColWaves / kNbrWaves;  // list-build workgroups per work item
 
 // InWhichCell (ExecPointLocator.h:271-274): unclamped trunc
 __device__ __forceinline__ int cell_axis(double x, double lo, double d) {

@@ -229,11 +229,14 @@ void launch_gather(const SysView& s, hipStream_t st) {
   hipLaunchKernelGGL(k_gather, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
 }
 
-// VERLET list build.  One workgroup per work item (64 i-atoms of one cell);
-// wavefront w = 3 (a+1) + (b+1) owns the column (a, b) of the 3x3x3 stencil
-// (its three cells e = -1, 0, 1) and writes list segment w, so the
-// concatenated segments follow the reference's order (x offset outermost,
-// then y, then z, then the cell's sorted order).
+// VERLET list build.  Three workgroups per work item (64 i-atoms of one
+// cell), three wavefronts each: wavefront w = 3 (a+1) + (b+1) of the nine owns
+// the column (a, b) of the 3x3x3 stencil (its three cells e = -1, 0, 1) and
+// writes list segment w, so the concatenated segments follow the reference's
+// order (x offset outermost, then y, then z, then the cell's sorted order).
+// Small workgroups spread evenly over the CUs: with one 9-wavefront
+// workgroup per item, a 30,000-atom launch put three items on some CUs and
+// two on others, and the fullest CUs set the kernel time.
 //
 // The three cells of a column form one candidate stream.  Each round, every
 // lane holds one candidate (coalesced load, image shift applied; the next
@@ -251,13 +254,13 @@ void launch_gather(const SysView& s, hipStream_t st) {
 // not the binned one, e.g. after SHAKE): one workgroup per misfit atom, the 9
 // wavefronts again own the 9 columns, lanes test 64 candidates at a time and
 // accepted ones are written at their ballot prefix position (stream order).
-constexpr int kColWaves = 9;
-
-__device__ void misfit_block(const SysView& s, int wave, int lane, int bid) {
+// wavefront `wave` (its stencil column) of misfit slot `slot`: misfit atoms
+// slot, slot + kMisfitBlocks, ...
+__device__ void misfit_block(const SysView& s, int wave, int lane, int slot) {
   const int count = s.misfit_count[0];
   const int a = wave / 3 - 1, b = wave % 3 - 1;
   const int seg0 = wave * s.seg_cap;
-  for (int m = bid - s.n_items; m < count; m += kMisfitBlocks) {
+  for (int m = slot; m < count; m += kMisfitBlocks) {
     const int i = __builtin_amdgcn_readfirstlane(s.misfit_list[m]);
     const double4 pi = s.spos[i];
     const int ca = cell_axis(pi.x, s.lo[0], s.dxdydz[0]);
@@ -292,15 +295,19 @@ __device__ void misfit_block(const SysView& s, int wave, int lane, int bid) {
   }
 }
 
+// One workgroup: kNbrWaves columns of one work item (or of one misfit slot);
+// block bid is part bid % kNbrParts of work item bid / kNbrParts.
 __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const int bid) {
-  __shared__ double4 stage[kColWaves][kWave];  // position, entry in the low word of w
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  __shared__ double4 stage[kNbrWaves][kWave];  // position, entry in the low word of w
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
-  if (bid >= s.n_items) {
-    misfit_block(s, wave, lane, bid);
+  const int wid = bid / kNbrParts;
+  const int wave = (bid % kNbrParts) * kNbrWaves + w;  // the stencil column this wavefront owns
+  if (wid >= s.n_items) {
+    misfit_block(s, wave, lane, wid - s.n_items);
     return;
   }
-  const int item_idx = bid;
+  const int item_idx = wid;
   if (item_idx >= s.n_items_dev[0]) return;
   const int item = __builtin_amdgcn_readfirstlane(s.items[item_idx]);
   const int c = item >> 8;
@@ -365,7 +372,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
     if (kc == 0) continue;
     if (keep) {
       int slot = __popcll(bal & ((1ull << lane) - 1ull));
-      stage[wave][slot] = make_double4(pj.x, pj.y, pj.z, __hiloint2double(0, entry));
+      stage[w][slot] = make_double4(pj.x, pj.y, pj.z, __hiloint2double(0, entry));
     }
     wave_sync();
     // test every staged candidate; an accepting lane appends the candidate's
@@ -376,7 +383,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
     for (; u + 4 <= kc; u += 4) {
       double4 q[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) q[r] = stage[wave][u + r];
+      for (int r = 0; r < 4; ++r) q[r] = stage[w][u + r];
       bool ok[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -395,7 +402,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
       }
     }
     for (; u < kc; ++u) {
-      const double4 q = stage[wave][u];
+      const double4 q = stage[w][u];
       const double rx = q.x - pi.x;
       const double ry = q.y - pi.y;
       const double rz = q.z - pi.z;
@@ -411,7 +418,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   if (active) s.nbr_num[wave * s.n + i] = cnt < seg_cap ? cnt : seg_cap;
 }
 
-__global__ __launch_bounds__(kColWaves * kWave) void k_nbr_build_verlet(SysView s) {
+__global__ __launch_bounds__(kNbrWaves * kWave) void k_nbr_build_verlet(SysView s) {
   nbr_build_verlet_body(s, (int)blockIdx.x);
 }
 
@@ -570,7 +577,8 @@ void launch_nbr_build(const SysView& s, hipStream_t st) {
     int waves = s.n < 1024 ? s.n : 1024;
     hipLaunchKernelGGL(k_nbr_misfit_rbl, dim3((waves * kWave + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
   } else {
-    hipLaunchKernelGGL(k_nbr_build_verlet, dim3(s.n_items + kMisfitBlocks), dim3(kColWaves * kWave), 0, st, s);
+    hipLaunchKernelGGL(k_nbr_build_verlet, dim3(kNbrParts * (s.n_items + kMisfitBlocks)), dim3(kNbrWaves * kWave), 0,
+                       st, s);
   }
 }
 
@@ -596,7 +604,7 @@ __global__ __launch_bounds__(kBlock) void kp_gather(PACK_PARAMS) {
   gather_body(e.v, bid);
 }
 
-__global__ __launch_bounds__(kColWaves * kWave) void kp_nbr_build(PACK_PARAMS) {
+__global__ __launch_bounds__(kNbrWaves * kWave) void kp_nbr_build(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
   nbr_build_verlet_body(e.v, bid);
 }
@@ -618,7 +626,7 @@ void launch_pack_cells(int kernel, const PackLaunch& L, int nblocks, size_t lds,
       hipLaunchKernelGGL(kp_gather, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
       break;
     case PK_NBR:
-      hipLaunchKernelGGL(kp_nbr_build, grid, dim3(kColWaves * kWave), 0, st, PACK_ARGS(L));
+      hipLaunchKernelGGL(kp_nbr_build, grid, dim3(kNbrWaves * kWave), 0, st, PACK_ARGS(L));
       break;
     default:
       break;

@@ -380,7 +380,7 @@ int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel) {
     case PK_RATTLE:
       return mols;
     case PK_NBR:
-      return s.n_items + kMisfitBlocks;
+      return kNbrParts * (s.n_items + kMisfitBlocks);
     case PK_PAIR:
       return pair_blocks(s);
     case PK_RHO:
