@@ -1743,6 +1743,7 @@ struct rbmd_pack {
   int nblk[PK_COUNT] = {};
   size_t lds_first = 0, lds_scatter = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // captured packed step without / with thermo sums
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // side-stream branch of a packed step
   std::vector<void*> allocs;
 };
 
@@ -1772,38 +1773,52 @@ static void pack_enqueue_step(rbmd_pack* p, bool thermo) {
   hipStream_t st = c->stream;
   PackSig sig = p->sig;
   sig.thermo = thermo ? 1 : 0;
-  auto integ = [&](int k, int arg, size_t lds) {
-    c->run(kPackName[k], [&] { launch_pack_integrate(k, p->L[k], p->nblk[k], sig, arg, lds, nullptr, st); });
+  auto integ = [&](int k, int arg, size_t lds, hipStream_t s) {
+    c->run(kPackName[k], [&] { launch_pack_integrate(k, p->L[k], p->nblk[k], sig, arg, lds, nullptr, s); });
   };
   auto cells = [&](int k, size_t lds) {
     c->run(kPackName[k], [&] { launch_pack_cells(k, p->L[k], p->nblk[k], lds, st); });
   };
-  auto ksp = [&](int k) { c->run(kPackName[k], [&] { launch_pack_kspace(k, p->L[k], p->nblk[k], sig, st); }); };
-  integ(PK_NHC, p->nh ? 1 : 5, 0);
-  if (p->ring) ksp(PK_KSLOT);
-  integ(PK_FIRST_HALF, p->tc == RBMD_TEMP_NOSE_HOOVER ? 1 : 0, p->lds_first);
+  auto ksp = [&](int k, hipStream_t s) {
+    c->run(kPackName[k], [&] { launch_pack_kspace(k, p->L[k], p->nblk[k], sig, s); });
+  };
+  integ(PK_NHC, p->nh ? 1 : 5, 0, st);
+  if (p->ring) ksp(PK_KSLOT, st);
+  integ(PK_FIRST_HALF, p->tc == RBMD_TEMP_NOSE_HOOVER ? 1 : 0, p->lds_first, st);
   cells(PK_SCAN, 0);
   cells(PK_SCATTER, p->lds_scatter);
   cells(PK_CELL_SORT, 0);
-  if (p->shake) integ(PK_SHAKE, 0, 0);
+  if (p->shake) integ(PK_SHAKE, 0, 0, st);
+  // as in a single system's step: the k-space and bonded forces branch off to
+  // the side stream once the positions are final (one stream when profiling)
+  const bool fork = !c->prof && (p->has_k || p->has_b);
+  const hipStream_t sk = fork ? c->side : st;
+  if (fork) {
+    (void)hipEventRecord(p->ev_fork, st);
+    (void)hipStreamWaitEvent(sk, p->ev_fork, 0);
+  }
+  if (p->has_k) {
+    ksp(PK_RHO, sk);
+    ksp(PK_RHO_RED, sk);
+    ksp(PK_KFORCE, sk);
+  }
+  if (p->has_b) integ(PK_BONDED, 0, 0, sk);
+  if (fork) (void)hipEventRecord(p->ev_join, sk);
   cells(PK_GATHER, 0);
   cells(PK_NBR, 0);
   c->run(kPackName[PK_PAIR], [&] { launch_pack_pair(p->L[PK_PAIR], p->nblk[PK_PAIR], sig, st); });
-  if (p->has_k) {
-    ksp(PK_RHO);
-    ksp(PK_RHO_RED);
-    ksp(PK_KFORCE);
-  }
-  if (p->has_b) integ(PK_BONDED, 0, 0);
-  integ(PK_SECOND, 1, 0);
-  integ(PK_NHC, 2 | 8, 0);
-  if (p->scale_v) integ(PK_ATOMS, 0, 0);
-  if (p->shake) integ(PK_RATTLE, 0, 0);
+  if (fork) (void)hipStreamWaitEvent(st, p->ev_join, 0);
+  integ(PK_SECOND, 1, 0, st);
+  integ(PK_NHC, 2 | 8, 0, st);
+  if (p->scale_v) integ(PK_ATOMS, 0, 0, st);
+  if (p->shake) integ(PK_RATTLE, 0, 0, st);
 }
 
 static rbmd_status pack_fail_free(rbmd_pack* p, rbmd_status rc) {
   for (void* q : p->allocs) (void)hipFree(q);
   if (p->hstatus) (void)hipHostFree(p->hstatus);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
   delete p;
   return rc;
 }
@@ -1930,6 +1945,9 @@ rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n,
   if (rc != RBMD_OK) return pack_fail_free(p, rc);
   if (hipHostMalloc((void**)&p->hstatus, sizeof(int) * n) != hipSuccess)
     return pack_fail_free(p, fail(RBMD_ERR_OOM, "pinned status array"));
+  if (hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess)
+    return pack_fail_free(p, fail(RBMD_ERR_HIP, "hipEventCreate"));
   hipStream_t st = ctx->stream;
   if (hipMemcpyAsync(p->dsys, tab.data(), sizeof(PackSys) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(p->dmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
