@@ -124,15 +124,34 @@ int rho_kblocks(const SysView& s, int P) { return (P + s.rho_kb - 1) / s.rho_kb;
     }                                            \
   } while (0)
 
+// rho[t] = sum over the partial rows b of rho_part[b][t].  One workgroup per
+// kRedCols columns (one per lane, coalesced rows); wavefront w sums rows w,
+// w + kRedWaves, ... (independent loads, in flight together), and the
+// kRedWaves partial sums are added in wavefront order: a fixed order, and
+// ~300 rows at 300,000 atoms take ~20 loads per wavefront instead of one
+// thread's chain of 300.
+constexpr int kRedCols = kWave;
+constexpr int kRedWaves = 16;
+
 __device__ __forceinline__ void rho_reduce_body(const KView& kv, const int bid) {
-  int t = bid * blockDim.x + threadIdx.x;
-  if (t >= 2 * kv.P) return;
+  __shared__ double part[kRedWaves][kRedCols];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const int n2 = 2 * kv.P;
+  const int t = bid * kRedCols + lane;
   double acc = 0.0;
-  for (int b = 0; b < kv.nblk_rho; ++b) acc += kv.rho_part[(size_t)b * 2 * kv.P + t];
-  kv.rho[t] = acc;
+  if (t < n2)
+    for (int b = w; b < kv.nblk_rho; b += kRedWaves) acc += kv.rho_part[(size_t)b * n2 + t];
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && t < n2) {
+    double r = part[0][lane];
+    for (int v = 1; v < kRedWaves; ++v) r += part[v][lane];
+    kv.rho[t] = r;
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_rho_reduce(KView kv) {
+__global__ __launch_bounds__(kRedWaves * kWave) void k_rho_reduce(KView kv) {
   rho_reduce_body(kv, (int)blockIdx.x);
 }
 
@@ -162,7 +181,7 @@ void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st) {
 }
 
 void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st) {
-  hipLaunchKernelGGL(k_rho_reduce, dim3((2 * k.P + kBlock - 1) / kBlock), dim3(kBlock), 0, st, k);
+  hipLaunchKernelGGL(k_rho_reduce, dim3((2 * k.P + kRedCols - 1) / kRedCols), dim3(kRedWaves * kWave), 0, st, k);
 }
 
 constexpr int kForceTile = 256;
@@ -307,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void kp_rho_partial(PACK_PARAMS) {
   rho_partial_body<F32, KB>(e.v, e.kv, bid % kb, bid / kb);
 }
 
-__global__ __launch_bounds__(kBlock) void kp_rho_reduce(PACK_PARAMS) {
+__global__ __launch_bounds__(kRedWaves * kWave) void kp_rho_reduce(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
   rho_reduce_body(e.kv, bid);
 }
@@ -331,7 +350,7 @@ void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const Pack
 #undef KP_RHO_LAUNCH
       break;
     case PK_RHO_RED:
-      hipLaunchKernelGGL(kp_rho_reduce, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
+      hipLaunchKernelGGL(kp_rho_reduce, grid, dim3(kRedWaves * kWave), 0, st, PACK_ARGS(L));
       break;
     case PK_KFORCE:
 #define KP_KF_LAUNCH(F32_, LG_)                                                                      \
@@ -386,7 +405,7 @@ int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel) {
     case PK_RHO:
       return rho_kblocks(s, P) * ((s.n + kBlock * kRhoAtomsPerThread - 1) / (kBlock * kRhoAtomsPerThread));
     case PK_RHO_RED:
-      return (2 * P + kBlock - 1) / kBlock;
+      return (2 * P + kRedCols - 1) / kRedCols;
     case PK_KFORCE:
       return kforce_blocks(s);
     default:
