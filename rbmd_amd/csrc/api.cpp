@@ -151,7 +151,7 @@ struct rbmd_sys {
   int n = 0, ntypes = 0, nb = 0, na = 0, nmol = 0;
   SysView v;
   IntView iv;
-  KView kv;
+  KView kv{};
   std::vector<void*> allocs;
   int64_t bytes = 0;
   // partial-sum rows, one segment per producer; seg_used = rows its last
@@ -187,6 +187,7 @@ struct rbmd_sys {
   bool injected = false;
   std::vector<double> k_injected;
   double* rho_part = nullptr;
+  KRec* krec = nullptr;  // [P] k records of the force pass
   double* rho = nullptr;
   int rho_part_cap = 0;  // doubles
   // host mirrors
@@ -450,11 +451,13 @@ rbmd_status ensure_rho(rbmd_sys* s, int P) {
   if ((size_t)s->rho_part_cap < need) {
     RC_TRY(s->dalloc(&s->rho_part, need));
     RC_TRY(s->dalloc(&s->rho, 2 * (size_t)P));
+    RC_TRY(s->dalloc(&s->krec, (size_t)P));
     s->rho_part_cap = (int)need;
   }
   s->kv.nblk_rho = nblk;
   s->kv.rho_part = s->rho_part;
   s->kv.rho = s->rho;
+  s->kv.krec = s->krec;
   return RBMD_OK;
 }
 
@@ -470,11 +473,11 @@ static void launch_k_bonded(rbmd_sys* s, const double* kvec, int thermo, hipStre
     k.kvec = kvec;
     k.S_over_P = s->p.coulomb_type == RBMD_COULOMB_RBE ? s->S / (double)s->P : 1.0;
     k.f32 = s->v.mixed;
-    int ewald = s->p.coulomb_type == RBMD_COULOMB_EWALD;
+    k.ewald = s->p.coulomb_type == RBMD_COULOMB_EWALD;
     c->run("rho_partial", [&] { launch_rho_partial(s->v, k, st); });
     c->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, st); });
     int nblk = 0;
-    c->run("kforce", [&] { launch_kforce(s->v, k, st, ewald, s->p.alpha, thermo, 0, s->seg_rows(SEG_K), &nblk); });
+    c->run("kforce", [&] { launch_kforce(s->v, k, st, thermo, 0, s->seg_rows(SEG_K), &nblk); });
     if (thermo) s->seg_used[SEG_K] = nblk;
   }
   if (has_bonded(s)) {
@@ -598,7 +601,7 @@ rbmd_status long_energy(rbmd_sys* s, const int kmax[3], double* e_avg, double* g
   HIP_TRY(hipMalloc(&rp, (size_t)nblk * 2 * K * sizeof(double)));
   HIP_TRY(hipMalloc(&rr, (size_t)2 * K * sizeof(double)));
   HIP_TRY(hipMemcpyAsync(kdev, kg.data(), kg.size() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
-  KView k{K, 1.0, kdev, rp, rr, nblk};
+  KView k{K, 1.0, kdev, rp, rr, nblk, 0, nullptr, 1};  // energy only: no force records
   launch_rho_partial(s->v, k, s->ctx->stream);
   launch_rho_reduce(s->v, k, s->ctx->stream);
   std::vector<double> rho((size_t)2 * K);
@@ -1370,11 +1373,12 @@ static rbmd_status kforce_out(rbmd_sys* s, int ewald, double* force, double* vir
   k.kvec = s->kcur;
   k.S_over_P = ewald ? 1.0 : s->S / (double)s->P;
   k.f32 = ewald ? 0 : s->v.mixed;  // the converged Ewald sum is always FP64
+  k.ewald = ewald;
   hipStream_t st = s->ctx->stream;
   s->ctx->run("rho_partial", [&] { launch_rho_partial(s->v, k, st); });
   s->ctx->run("rho_reduce", [&] { launch_rho_reduce(s->v, k, st); });
   s->ctx->run("kforce", [&] {
-    launch_kforce(s->v, k, st, ewald, s->p.alpha, 1, 1, s->seg_rows(SEG_K), &s->seg_used[SEG_K]);
+    launch_kforce(s->v, k, st, 1, 1, s->seg_rows(SEG_K), &s->seg_used[SEG_K]);
   });
   RC_TRY(check_launch("kforce"));
   int n = s->n;
@@ -1896,6 +1900,7 @@ rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n,
     e.kv.kvec = p->ring ? s->kgraph : s->kcur;
     e.kv.S_over_P = s->p.coulomb_type == RBMD_COULOMB_RBE ? s->S / (double)s->P : 1.0;
     e.kv.f32 = s->v.mixed;
+    e.kv.ewald = s->p.coulomb_type == RBMD_COULOMB_EWALD;
     e.nhc = s->nhc;
     e.np = s->np;
     e.rd = pack_rows_desc(s, p->has_k, p->has_b, p->shake);

@@ -161,6 +161,13 @@ void launch_force_cells(const SysView& s, hipStream_t st, int thermo, int perato
 int force_cells_blocks(const SysView& s);
 
 // ---------------------------------------------------------------- k space
+// One k vector as the force pass reads it (written by the rho reduction):
+// k, the prefactor 1/(V k^2) (Ewald: times e^{-k^2/4 alpha}), m = k L / 2 pi
+// and rho(k).  80 bytes, read with scalar loads.
+struct KRec {
+  double kx, ky, kz, pre, mx, my, mz, rr, ri, pad;
+};
+
 struct KView {
   int P;
   double S_over_P;      // S / P
@@ -169,11 +176,13 @@ struct KView {
   double* rho;          // [2P]
   int nblk_rho;
   int f32;              // phases' sin/cos and the force coefficient in FP32 (sums in FP64)
+  KRec* krec;           // [P] records for the force pass (null: the reduction writes none)
+  int ewald;            // prefactor with the Ewald Gaussian
 };
 void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st);
 void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st);
-void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, int thermo,
-                   int peratom, double* partials, int* nblocks);
+void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int thermo, int peratom, double* partials,
+                   int* nblocks);
 int kforce_blocks(const SysView& s);
 void launch_kvec_slot(double* dst, const double* ring, int R, int per, const int64_t* step, hipStream_t st);
 

@@ -133,7 +133,8 @@ int rho_kblocks(const SysView& s, int P) { return (P + s.rho_kb - 1) / s.rho_kb;
 constexpr int kRedCols = kWave;
 constexpr int kRedWaves = 16;
 
-__device__ __forceinline__ void rho_reduce_body(const KView& kv, const int bid) {
+__device__ __forceinline__ void rho_reduce_body(const SysView& s, const KView& kv, const int bid) {
+#pragma clang fp contract(off)
   __shared__ double part[kRedWaves][kRedCols];
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
@@ -148,11 +149,30 @@ __device__ __forceinline__ void rho_reduce_body(const KView& kv, const int bid) 
     double r = part[0][lane];
     for (int v = 1; v < kRedWaves; ++v) r += part[v][lane];
     kv.rho[t] = r;
+    if (kv.krec) {
+      KRec& rec = kv.krec[t >> 1];
+      if (t & 1) {
+        rec.ri = r;
+      } else {
+        const int l = t >> 1;
+        const double kx = kv.kvec[3 * l], ky = kv.kvec[3 * l + 1], kz = kv.kvec[3 * l + 2];
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        rec.kx = kx;
+        rec.ky = ky;
+        rec.kz = kz;
+        rec.pre = kv.ewald ? exp(-k2 / (4 * s.alpha)) / (s.volume * k2) : 1.0 / (s.volume * k2);
+        rec.mx = k_to_m(kx, s.box[0]);
+        rec.my = k_to_m(ky, s.box[1]);
+        rec.mz = k_to_m(kz, s.box[2]);
+        rec.rr = r;
+        rec.pad = 0.0;
+      }
+    }
   }
 }
 
-__global__ __launch_bounds__(kRedWaves * kWave) void k_rho_reduce(KView kv) {
-  rho_reduce_body(kv, (int)blockIdx.x);
+__global__ __launch_bounds__(kRedWaves * kWave) void k_rho_reduce(SysView s, KView kv) {
+  rho_reduce_body(s, kv, (int)blockIdx.x);
 }
 
 // the k vectors of the current step (device step counter) from the
@@ -181,25 +201,24 @@ void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st) {
 }
 
 void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st) {
-  hipLaunchKernelGGL(k_rho_reduce, dim3((2 * k.P + kRedCols - 1) / kRedCols), dim3(kRedWaves * kWave), 0, st, k);
+  hipLaunchKernelGGL(k_rho_reduce, dim3((2 * k.P + kRedCols - 1) / kRedCols), dim3(kRedWaves * kWave), 0, st, s, k);
 }
 
-constexpr int kForceTile = 256;
-
-// prefactor pre_l = -4 pi / (V k^2) [* e^{-k^2/4alpha} for Ewald]; the 2^LG
-// lanes of an atom (a tile choice, SysView::k_lanes_lg) split each LDS tile:
-// lane `sub` takes k-vectors sub, sub + 2^LG, ...
+// Force pass.  One workgroup: 64 atoms (one per lane) against all P k
+// vectors, split into KS = 2^LG contiguous ranges, one per wavefront (a tile
+// choice, SysView::k_lanes_lg).  The lanes of a wavefront take the same k
+// vector at the same time, so its record (KRec) is wave-uniform and comes
+// through scalar loads; nothing of k passes through LDS.  The wavefronts'
+// partial sums are added in wavefront order.
 template <bool THERMO, bool PERATOM, bool F32, int LG>
-__device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, int ewald, double alpha, double* rows, const int bid) {
+__device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, double* rows, const int bid) {
 #pragma clang fp contract(off)  // fused multiply-adds are written out: the same forces with or without the virial sums
-  constexpr int kKLg = LG;
-  constexpr int L = 1 << kKLg;
-  __shared__ double tk[kForceTile][4];
-  __shared__ double tm[kForceTile][3];
-  __shared__ double tr[kForceTile][2];
-  const int t = bid * blockDim.x + threadIdx.x;
-  const int i = t >> kKLg;
-  const int sub = t & (L - 1);
+  constexpr int KS = 1 << LG;
+  constexpr int NX = PERATOM ? 9 : 3;  // per-atom sums the wavefronts hand over
+  __shared__ double part[KS][NX][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int i = bid * kWave + lane;
   const bool act = i < s.n;
   double x = 0, y = 0, z = 0, q = 0, sx = 0, sy = 0, sz = 0;
   if (act) {
@@ -211,60 +230,55 @@ __device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, i
     sy = y * (1.0 / s.box[1]);
     sz = z * (1.0 / s.box[2]);
   }
+  const double q4 = -4.0 * kPi * q;
   double fx = 0, fy = 0, fz = 0;
   double v[6] = {0, 0, 0, 0, 0, 0};
-  for (int base = 0; base < kv.P; base += kForceTile) {
-    __syncthreads();
-    for (int u = threadIdx.x; u < kForceTile; u += blockDim.x) {
-      int l = base + u;
-      if (l < kv.P) {
-        double kx = kv.kvec[3 * l], ky = kv.kvec[3 * l + 1], kz = kv.kvec[3 * l + 2];
-        double k2 = kx * kx + ky * ky + kz * kz;
-        tk[u][0] = kx;
-        tk[u][1] = ky;
-        tk[u][2] = kz;
-        tk[u][3] = ewald ? exp(-k2 / (4 * alpha)) / (s.volume * k2) : 1.0 / (s.volume * k2);
-        tm[u][0] = k_to_m(kx, s.box[0]);
-        tm[u][1] = k_to_m(ky, s.box[1]);
-        tm[u][2] = k_to_m(kz, s.box[2]);
-        tr[u][0] = kv.rho[2 * l];
-        tr[u][1] = kv.rho[2 * l + 1];
-      }
-    }
-    __syncthreads();
-    int cnt = kv.P - base < kForceTile ? kv.P - base : kForceTile;
-    for (int u = sub; u < cnt; u += L) {
-      double kx = tk[u][0], ky = tk[u][1], kz = tk[u][2];
-      double sn, cs;
-      phase_sincos<F32>(tm[u][0], tm[u][1], tm[u][2], sx, sy, sz, &sn, &cs);
-      const double fs = (-4.0 * kPi * q) * tk[u][3] * fma(cs, tr[u][1], -(sn * tr[u][0]));
-      fx = fma(fs, kx, fx);
-      fy = fma(fs, ky, fy);
-      fz = fma(fs, kz, fz);
-      if (THERMO || PERATOM) {
-        v[0] += (x * kx + x * kx) * fs;
-        v[1] += (y * ky + y * ky) * fs;
-        v[2] += (z * kz + z * kz) * fs;
-        v[3] += (x * ky + y * kx) * fs;
-        v[4] += (x * kz + z * kx) * fs;
-        v[5] += (y * kz + z * ky) * fs;
-      }
+  const int per = (kv.P + KS - 1) / KS;
+  const int l0 = w * per;
+  const int l1 = l0 + per < kv.P ? l0 + per : kv.P;
+  // constant address space: wave-uniform scalar loads (the records are not
+  // written during this kernel)
+  typedef const __attribute__((address_space(4))) KRec const_krec;
+  const const_krec* rec = (const const_krec*)kv.krec;
+  for (int l = l0; l < l1; ++l) {
+    const double kx = rec[l].kx, ky = rec[l].ky, kz = rec[l].kz;
+    double sn, cs;
+    phase_sincos<F32>(rec[l].mx, rec[l].my, rec[l].mz, sx, sy, sz, &sn, &cs);
+    const double fs = (q4 * rec[l].pre) * fma(cs, rec[l].ri, -(sn * rec[l].rr));
+    fx = fma(fs, kx, fx);
+    fy = fma(fs, ky, fy);
+    fz = fma(fs, kz, fz);
+    if (THERMO || PERATOM) {
+      v[0] += (x * kx + x * kx) * fs;
+      v[1] += (y * ky + y * ky) * fs;
+      v[2] += (z * kz + z * kz) * fs;
+      v[3] += (x * ky + y * kx) * fs;
+      v[4] += (x * kz + z * kx) * fs;
+      v[5] += (y * kz + z * ky) * fs;
     }
   }
   const double sc = kv.S_over_P;
   if (THERMO) {
     const double vals[kRowK] = {v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc, v[4] * sc, v[5] * sc};
-    block_write_row<kRowK>(vals, rows + (size_t)bid * kRowK);
+    block_write_row<kRowK, KS>(vals, rows + (size_t)bid * kRowK);
   }
-#pragma unroll
-  for (int off = 1; off < L; off <<= 1) {
-    fx += __shfl_xor(fx, off, kWave);
-    fy += __shfl_xor(fy, off, kWave);
-    fz += __shfl_xor(fz, off, kWave);
+  // the atom's totals: wavefront partials added in order
+  if (w > 0) {
+    part[w][0][lane] = fx;
+    part[w][1][lane] = fy;
+    part[w][2][lane] = fz;
     if (PERATOM)
-      for (int k = 0; k < 6; ++k) v[k] += __shfl_xor(v[k], off, kWave);
+      for (int k = 0; k < 6; ++k) part[w][3 + k][lane] = v[k];
   }
-  if (act && sub == 0) {
+  __syncthreads();
+  if (w == 0 && act) {
+    for (int u = 1; u < KS; ++u) {
+      fx += part[u][0][lane];
+      fy += part[u][1][lane];
+      fz += part[u][2][lane];
+      if (PERATOM)
+        for (int k = 0; k < 6; ++k) v[k] += part[u][3 + k][lane];
+    }
     s.fkx[i] = s.qqr2e * (fx * sc);
     s.fky[i] = s.qqr2e * (fy * sc);
     s.fkz[i] = s.qqr2e * (fz * sc);
@@ -274,42 +288,41 @@ __device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, i
 }
 
 template <bool THERMO, bool PERATOM, bool F32, int LG>
-__global__ __launch_bounds__(kBlock) void k_kforce(SysView s, KView kv, int ewald, double alpha, double* rows) {
-  kforce_body<THERMO, PERATOM, F32, LG>(s, kv, ewald, alpha, rows, (int)blockIdx.x);
+__global__ __launch_bounds__(kWave << LG) void k_kforce(SysView s, KView kv, double* rows) {
+  kforce_body<THERMO, PERATOM, F32, LG>(s, kv, rows, (int)blockIdx.x);
 }
 
-int kforce_blocks(const SysView& s) { return (int)((((size_t)s.n << s.k_lanes_lg) + kBlock - 1) / kBlock); }
+int kforce_blocks(const SysView& s) { return (s.n + kWave - 1) / kWave; }
 
 template <bool THERMO, bool PERATOM, bool F32>
-void launch_kforce_lg(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, double* rows,
-                      int grid) {
+void launch_kforce_lg(const SysView& s, const KView& k, hipStream_t st, double* rows, int grid) {
   if (s.k_lanes_lg == 1)
-    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 1>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 1>), dim3(grid), dim3(kWave << 1), 0, st, s, k, rows);
   else if (s.k_lanes_lg == 3)
-    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 3>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 3>), dim3(grid), dim3(kWave << 3), 0, st, s, k, rows);
   else
-    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 2>), dim3(grid), dim3(kBlock), 0, st, s, k, ewald, alpha, rows);
+    hipLaunchKernelGGL((k_kforce<THERMO, PERATOM, F32, 2>), dim3(grid), dim3(kWave << 2), 0, st, s, k, rows);
 }
 
-void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int ewald, double alpha, int thermo, int peratom,
-                   double* rows, int* nblocks) {
+void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int thermo, int peratom, double* rows,
+                   int* nblocks) {
   int grid = kforce_blocks(s);
   if (nblocks) *nblocks = grid;
   if (k.f32) {
     if (peratom)
-      launch_kforce_lg<true, true, true>(s, k, st, ewald, alpha, rows, grid);
+      launch_kforce_lg<true, true, true>(s, k, st, rows, grid);
     else if (thermo)
-      launch_kforce_lg<true, false, true>(s, k, st, ewald, alpha, rows, grid);
+      launch_kforce_lg<true, false, true>(s, k, st, rows, grid);
     else
-      launch_kforce_lg<false, false, true>(s, k, st, ewald, alpha, rows, grid);
+      launch_kforce_lg<false, false, true>(s, k, st, rows, grid);
     return;
   }
   if (peratom)
-    launch_kforce_lg<true, true, false>(s, k, st, ewald, alpha, rows, grid);
+    launch_kforce_lg<true, true, false>(s, k, st, rows, grid);
   else if (thermo)
-    launch_kforce_lg<true, false, false>(s, k, st, ewald, alpha, rows, grid);
+    launch_kforce_lg<true, false, false>(s, k, st, rows, grid);
   else
-    launch_kforce_lg<false, false, false>(s, k, st, ewald, alpha, rows, grid);
+    launch_kforce_lg<false, false, false>(s, k, st, rows, grid);
 }
 
 // ------------------------------------------------------------------ packs
@@ -328,13 +341,13 @@ __global__ __launch_bounds__(kBlock) void kp_rho_partial(PACK_PARAMS) {
 
 __global__ __launch_bounds__(kRedWaves * kWave) void kp_rho_reduce(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
-  rho_reduce_body(e.kv, bid);
+  rho_reduce_body(e.v, e.kv, bid);
 }
 
 template <bool F32, int LG, bool THERMO>
-__global__ __launch_bounds__(kBlock) void kp_kforce(PACK_PARAMS, int ewald) {
+__global__ __launch_bounds__(kWave << LG) void kp_kforce(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
-  kforce_body<THERMO, false, F32, LG>(e.v, e.kv, ewald, e.v.alpha, e.rows_k, bid);
+  kforce_body<THERMO, false, F32, LG>(e.v, e.kv, e.rows_k, bid);
 }
 
 void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const PackSig& g, hipStream_t st) {
@@ -353,12 +366,12 @@ void launch_pack_kspace(int kernel, const PackLaunch& L, int nblocks, const Pack
       hipLaunchKernelGGL(kp_rho_reduce, grid, dim3(kRedWaves * kWave), 0, st, PACK_ARGS(L));
       break;
     case PK_KFORCE:
-#define KP_KF_LAUNCH(F32_, LG_)                                                                      \
-  do {                                                                                               \
-    if (g.thermo)                                                                                    \
-      hipLaunchKernelGGL((kp_kforce<F32_, LG_, true>), grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald);  \
-    else                                                                                             \
-      hipLaunchKernelGGL((kp_kforce<F32_, LG_, false>), grid, dim3(kBlock), 0, st, PACK_ARGS(L), g.ewald); \
+#define KP_KF_LAUNCH(F32_, LG_)                                                                    \
+  do {                                                                                             \
+    if (g.thermo)                                                                                  \
+      hipLaunchKernelGGL((kp_kforce<F32_, LG_, true>), grid, dim3(kWave << LG_), 0, st, PACK_ARGS(L));  \
+    else                                                                                           \
+      hipLaunchKernelGGL((kp_kforce<F32_, LG_, false>), grid, dim3(kWave << LG_), 0, st, PACK_ARGS(L)); \
   } while (0)
       if (g.k_lanes_lg == 1) {
         if (g.f32) KP_KF_LAUNCH(true, 1); else KP_KF_LAUNCH(false, 1);
