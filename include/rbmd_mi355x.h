@@ -136,10 +136,11 @@ typedef struct rbmd_params {
   double rbl_rho;                 /* number density used by RBL capacities; 0 = N/V */
   int32_t pair_path;              /* VERLET real space: RBMD_PAIR_* */
   /* tile choices (0 = built-in default): lanes per atom of the pair kernel
-   * (4, 8, 16; default by atom count), lanes per atom of the k-space force
-   * kernel (2, 4, 8; default 4), k vectors per structure-factor workgroup
-   * (4, 8, 16; default 8).  rbmd_amd/tune.py measures them per kind,
-   * precision and size and rbmd_amd/tiles.json holds the result. */
+   * (4, 8, 16; default by atom count), wavefronts splitting the k vectors
+   * of the k-space force kernel (2, 4, 8; default 4), k vectors per lane of
+   * the structure-factor kernel (1, 2, 4; default 1).  rbmd_amd/tune.py
+   * measures them per kind, precision and size and rbmd_amd/tiles.json holds
+   * the result. */
   int32_t pair_lanes;
   int32_t kforce_lanes;
   int32_t rho_kblock;

@@ -446,7 +446,8 @@ bool has_bonded(const rbmd_sys* s) { return s->p.force_field == RBMD_FF_CVFF && 
 bool shake_on(const rbmd_sys* s) { return s->p.fix_shake == RBMD_SHAKE_TRUE && s->nmol > 0; }
 
 rbmd_status ensure_rho(rbmd_sys* s, int P) {
-  int nblk = (s->n + kBlock * 4 - 1) / (kBlock * 4);
+  const int atoms = rho_atoms_per_block(s->n, rho_kblocks(s->v, P));
+  const int nblk = (s->n + atoms - 1) / atoms;
   size_t need = (size_t)nblk * 2 * P;
   if ((size_t)s->rho_part_cap < need) {
     RC_TRY(s->dalloc(&s->rho_part, need));
@@ -455,6 +456,7 @@ rbmd_status ensure_rho(rbmd_sys* s, int P) {
     s->rho_part_cap = (int)need;
   }
   s->kv.nblk_rho = nblk;
+  s->kv.rho_atoms = atoms;
   s->kv.rho_part = s->rho_part;
   s->kv.rho = s->rho;
   s->kv.krec = s->krec;
@@ -597,11 +599,12 @@ rbmd_status long_energy(rbmd_sys* s, const int kmax[3], double* e_avg, double* g
   double* kdev = nullptr;
   HIP_TRY(hipMalloc(&kdev, kg.size() * sizeof(double)));
   double *rp = nullptr, *rr = nullptr;
-  int nblk = (s->n + kBlock * 4 - 1) / (kBlock * 4);
+  const int atoms = rho_atoms_per_block(s->n, rho_kblocks(s->v, K));
+  const int nblk = (s->n + atoms - 1) / atoms;
   HIP_TRY(hipMalloc(&rp, (size_t)nblk * 2 * K * sizeof(double)));
   HIP_TRY(hipMalloc(&rr, (size_t)2 * K * sizeof(double)));
   HIP_TRY(hipMemcpyAsync(kdev, kg.data(), kg.size() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
-  KView k{K, 1.0, kdev, rp, rr, nblk, 0, nullptr, 1};  // energy only: no force records
+  KView k{K, 1.0, kdev, rp, rr, nblk, 0, nullptr, 1, atoms};  // energy only: no force records
   launch_rho_partial(s->v, k, s->ctx->stream);
   launch_rho_reduce(s->v, k, s->ctx->stream);
   std::vector<double> rho((size_t)2 * K);
@@ -725,8 +728,8 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     return fail(RBMD_ERR_ARG, "pair_lanes must be 0, 4, 8 or 16");
   if (p->kforce_lanes != 0 && p->kforce_lanes != 2 && p->kforce_lanes != 4 && p->kforce_lanes != 8)
     return fail(RBMD_ERR_ARG, "kforce_lanes must be 0, 2, 4 or 8");
-  if (p->rho_kblock != 0 && p->rho_kblock != 4 && p->rho_kblock != 8 && p->rho_kblock != 16)
-    return fail(RBMD_ERR_ARG, "rho_kblock must be 0, 4, 8 or 16");
+  if (p->rho_kblock != 0 && p->rho_kblock != 1 && p->rho_kblock != 2 && p->rho_kblock != 4)
+    return fail(RBMD_ERR_ARG, "rho_kblock must be 0, 1, 2 or 4");
   if (p->coulomb_type == RBMD_COULOMB_RBE && p->rbe_p <= 0) return fail(RBMD_ERR_ARG, "coulomb_sample_num must be > 0");
   if (p->coulomb_type != RBMD_COULOMB_NONE && !(p->alpha > 0)) return fail(RBMD_ERR_ARG, "alpha must be > 0");
   if (d->special_offsets && d->special_ids && d->special_weights) {
@@ -786,7 +789,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   v.lanes_lg = n <= 60000 ? 4 : (n <= 250000 ? 3 : 2);
   if (p->pair_lanes) v.lanes_lg = p->pair_lanes == 4 ? 2 : (p->pair_lanes == 8 ? 3 : 4);
   v.k_lanes_lg = p->kforce_lanes ? (p->kforce_lanes == 2 ? 1 : (p->kforce_lanes == 8 ? 3 : 2)) : 2;
-  v.rho_kb = p->rho_kblock ? p->rho_kblock : 8;
+  v.rho_kb = p->rho_kblock ? p->rho_kblock : 1;
   v.sp_max = 0;
   if (d->special_offsets && d->special_ids && d->special_weights)
     for (int i = 0; i < n; ++i) v.sp_max = std::max(v.sp_max, d->special_offsets[i + 1] - d->special_offsets[i]);
@@ -1427,6 +1430,7 @@ rbmd_status rbmd_ewald_force(rbmd_sys* s, const int32_t kmax[3], double* force, 
   if (rc == RBMD_OK) rc = kforce_out(s, 1, force, vir);
   s->kcur = saved_k;
   s->P = saved_P;
+  if (rc == RBMD_OK) rc = ensure_rho(s, saved_P);  // the step's block sizes again
   (void)hipFree(kdev);
   if (rc != RBMD_OK) return rc;
   if (energy_total) {

@@ -178,7 +178,12 @@ struct KView {
   int f32;              // phases' sin/cos and the force coefficient in FP32 (sums in FP64)
   KRec* krec;           // [P] records for the force pass (null: the reduction writes none)
   int ewald;            // prefactor with the Ewald Gaussian
+  int rho_atoms;        // atoms per structure-factor workgroup (= n / nblk_rho rounded up to it)
 };
+// atoms per structure-factor workgroup for n atoms and kblocks k-vector
+// blocks: 1024, halved (down to 64) while the grid has under 2048 workgroups
+int rho_atoms_per_block(int n, int kblocks);
+int rho_kblocks(const SysView& s, int P);  // k-vector blocks of the structure-factor grid
 void launch_rho_partial(const SysView& s, const KView& k, hipStream_t st);
 void launch_rho_reduce(const SysView& s, const KView& k, hipStream_t st);
 void launch_kforce(const SysView& s, const KView& k, hipStream_t st, int thermo, int peratom, double* partials,

@@ -19,7 +19,6 @@
 
 namespace rbmd {
 
-constexpr int kRhoAtomsPerThread = 4;
 // k-vectors per workgroup in the rho pass: a tile choice (4, 8 or 16, SysView::rho_kb)
 
 // e^{i k.r} with k = 2 pi m / L is e^{2 pi i t} with the phase t = m.(r/L) in
@@ -46,60 +45,73 @@ __device__ __forceinline__ void phase_sincos(double mx, double my, double mz, do
 // reciprocal rounds to the same integer as the quotient
 __device__ __forceinline__ double k_to_m(double k, double box) { return rint(k * box * (0.5 / kPi)); }
 
-// bk: block of KB k vectors, ba: block of kBlock * kRhoAtomsPerThread atoms
+// Structure factor partials.  One workgroup: kWave * KB k vectors (KB per
+// lane, a tile choice, SysView::rho_kb) against kv.rho_atoms atoms (64 to
+// 1024, by system size: rho_atoms_per_block), which its kBlock / kWave
+// wavefronts split into contiguous ranges.
+// The lanes of a wavefront take the same atom at the same time, so the atom
+// is wave-uniform and comes through scalar loads; the wavefronts' partial
+// sums are added in wavefront order and written as row ba of rho_part.
+// bk: block of k vectors, ba: block of atoms.
 template <bool F32, int KB>
 __device__ __forceinline__ void rho_partial_body(const SysView& s, const KView& kv, const int bk, const int ba) {
 #pragma clang fp contract(off)  // fused multiply-adds are written out: the same rounding in every kernel
-  constexpr int kRhoK = KB;
-  __shared__ double lds[kBlock / kWave][2 * kRhoK];
-  const int l0 = bk * kRhoK;
-  double re[kRhoK], im[kRhoK];
+  constexpr int NW = kBlock / kWave;
+  __shared__ double part[NW][2 * KB][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int kb0 = bk * kWave * KB;
+  double mx[KB], my[KB], mz[KB], re[KB], im[KB];
 #pragma unroll
-  for (int t = 0; t < kRhoK; ++t) re[t] = im[t] = 0.0;
-  double mx[kRhoK], my[kRhoK], mz[kRhoK];
-#pragma unroll
-  for (int t = 0; t < kRhoK; ++t) {
-    int l = l0 + t;
-    bool ok = l < kv.P;
+  for (int t = 0; t < KB; ++t) {
+    const int l = kb0 + t * kWave + lane;
+    const bool ok = l < kv.P;
     mx[t] = ok ? k_to_m(kv.kvec[3 * l], s.box[0]) : 0.0;
     my[t] = ok ? k_to_m(kv.kvec[3 * l + 1], s.box[1]) : 0.0;
     mz[t] = ok ? k_to_m(kv.kvec[3 * l + 2], s.box[2]) : 0.0;
+    re[t] = im[t] = 0.0;
   }
-  const int base = ba * kBlock * kRhoAtomsPerThread;
   const double ibx = 1.0 / s.box[0], iby = 1.0 / s.box[1], ibz = 1.0 / s.box[2];
+  const int per = kv.rho_atoms / NW;
+  const int a0 = ba * kv.rho_atoms + w * per;
+  const int a1 = a0 + per < s.n ? a0 + per : s.n;
+  // constant address space: wave-uniform scalar loads (positions and charges
+  // are not written during this kernel)
+  typedef const __attribute__((address_space(4))) double const_double;
+  const const_double* X = (const const_double*)s.x;
+  const const_double* Y = (const const_double*)s.y;
+  const const_double* Z = (const const_double*)s.z;
+  const const_double* Q = (const const_double*)s.q;
+#pragma unroll 2
+  for (int i = a0; i < a1; ++i) {
+    const double sx = X[i] * ibx, sy = Y[i] * iby, sz = Z[i] * ibz, q = Q[i];
 #pragma unroll
-  for (int a = 0; a < kRhoAtomsPerThread; ++a) {
-    int i = base + a * kBlock + threadIdx.x;
-    if (i < s.n) {
-      double sx = s.x[i] * ibx, sy = s.y[i] * iby, sz = s.z[i] * ibz, q = s.q[i];
-#pragma unroll
-      for (int t = 0; t < kRhoK; ++t) {
-        double sn, cs;
-        phase_sincos<F32>(mx[t], my[t], mz[t], sx, sy, sz, &sn, &cs);
-        re[t] = fma(q, cs, re[t]);
-        im[t] = fma(q, sn, im[t]);
-      }
+    for (int t = 0; t < KB; ++t) {
+      double sn, cs;
+      phase_sincos<F32>(mx[t], my[t], mz[t], sx, sy, sz, &sn, &cs);
+      re[t] = fma(q, cs, re[t]);
+      im[t] = fma(q, sn, im[t]);
     }
   }
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
 #pragma unroll
-  for (int t = 0; t < kRhoK; ++t) {
-    double a = wave_sum(re[t]);
-    double b = wave_sum(im[t]);
-    if (lane == 0) {
-      lds[wave][2 * t] = a;
-      lds[wave][2 * t + 1] = b;
-    }
+  for (int t = 0; t < KB; ++t) {
+    part[w][2 * t][lane] = re[t];
+    part[w][2 * t + 1][lane] = im[t];
   }
   __syncthreads();
-  if (threadIdx.x < 2 * kRhoK) {
-    int t = threadIdx.x >> 1;
-    int l = l0 + t;
-    if (l < kv.P) {
-      double acc = 0.0;
-      for (int w = 0; w < kBlock / kWave; ++w) acc += lds[w][threadIdx.x];
-      kv.rho_part[(size_t)ba * 2 * kv.P + 2 * l + (threadIdx.x & 1)] = acc;
+  if (w == 0) {
+#pragma unroll
+    for (int t = 0; t < KB; ++t) {
+      const int l = kb0 + t * kWave + lane;
+      if (l < kv.P) {
+        double a = part[0][2 * t][lane], b = part[0][2 * t + 1][lane];
+        for (int u = 1; u < NW; ++u) {
+          a += part[u][2 * t][lane];
+          b += part[u][2 * t + 1][lane];
+        }
+        kv.rho_part[(size_t)ba * 2 * kv.P + 2 * l] = a;
+        kv.rho_part[(size_t)ba * 2 * kv.P + 2 * l + 1] = b;
+      }
     }
   }
 }
@@ -110,17 +122,23 @@ __global__ __launch_bounds__(kBlock) void k_rho_partial(SysView s, KView kv) {
   rho_partial_body<F32, KB>(s, kv, (int)blockIdx.x, (int)blockIdx.y);
 }
 
-int rho_kblocks(const SysView& s, int P) { return (P + s.rho_kb - 1) / s.rho_kb; }
+int rho_kblocks(const SysView& s, int P) { return (P + kWave * s.rho_kb - 1) / (kWave * s.rho_kb); }
 
-// instantiate F(F32, KB) for the tile choice kb (4, 8 or 16)
+int rho_atoms_per_block(int n, int kblocks) {
+  int a = 1024;
+  while (a > 64 && (long)kblocks * ((n + a - 1) / a) < 2048) a >>= 1;
+  return a;
+}
+
+// instantiate F(F32, KB) for the tile choice kb (1, 2 or 4 k vectors per lane)
 #define RHO_DISPATCH(kb, f32, F)                 \
   do {                                           \
-    if ((kb) == 4) {                             \
+    if ((kb) == 1) {                             \
+      if (f32) F(true, 1); else F(false, 1);     \
+    } else if ((kb) == 4) {                      \
       if (f32) F(true, 4); else F(false, 4);     \
-    } else if ((kb) == 16) {                     \
-      if (f32) F(true, 16); else F(false, 16);   \
     } else {                                     \
-      if (f32) F(true, 8); else F(false, 8);     \
+      if (f32) F(true, 2); else F(false, 2);     \
     }                                            \
   } while (0)
 
@@ -335,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void kp_kvec_slot(PACK_PARAMS) {
 template <bool F32, int KB>
 __global__ __launch_bounds__(kBlock) void kp_rho_partial(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
-  const int kb = (e.kv.P + KB - 1) / KB;
+  const int kb = (e.kv.P + kWave * KB - 1) / (kWave * KB);  // k-vector blocks (rho_kblocks)
   rho_partial_body<F32, KB>(e.v, e.kv, bid % kb, bid / kb);
 }
 
@@ -416,7 +434,8 @@ int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel) {
     case PK_PAIR:
       return pair_blocks(s);
     case PK_RHO:
-      return rho_kblocks(s, P) * ((s.n + kBlock * kRhoAtomsPerThread - 1) / (kBlock * kRhoAtomsPerThread));
+      return rho_kblocks(s, P) * ((s.n + rho_atoms_per_block(s.n, rho_kblocks(s, P)) - 1) /
+                                  rho_atoms_per_block(s.n, rho_kblocks(s, P)));
     case PK_RHO_RED:
       return (2 * P + kRedCols - 1) / kRedCols;
     case PK_KFORCE:

@@ -9,8 +9,8 @@ with P = 200 N / 30,000 k vectors, as the served loop runs it) and times, with
 HIP events on the library's stream, every option of
 
   pair_lanes    lanes per atom of the pair kernel          4, 8, 16
-  kforce_lanes  lanes per atom of the k-space force kernel 2, 4, 8
-  rho_kblock    k vectors per structure-factor workgroup   4, 8, 16
+  kforce_lanes  wavefronts splitting the k vectors of the k-space force kernel  2, 4, 8
+  rho_kblock    k vectors per lane of the structure-factor kernel               1, 2, 4
 
 (option i of all three in run i: the kernels are timed separately).  The pair
 lanes are chosen per size (a per-system runtime value); the two k-space
@@ -28,7 +28,7 @@ import os
 import sys
 import time
 
-OPTIONS = {"pair_lanes": (4, 8, 16), "kforce_lanes": (2, 4, 8), "rho_kblock": (4, 8, 16)}
+OPTIONS = {"pair_lanes": (4, 8, 16), "kforce_lanes": (2, 4, 8), "rho_kblock": (1, 2, 4)}
 KERNEL = {"pair_lanes": "pair", "kforce_lanes": "kforce", "rho_kblock": "rho_partial"}
 KINDS = {"salt": "LJ/CUT/COUL/LONG", "water": "CVFF"}
 

@@ -14,28 +14,28 @@ def test_choose_rule():
     assert e["pair_lanes"] == [[10000, 16], [2 ** 31 - 1, 8]]
     # kforce: relative sums 2+1.05 / 1+1.03 / 1.2+1 -> option 1 (4 lanes)
     assert e["kforce_lanes"] == 4
-    # rho: 1+1.15 / 1.33+1 / 2.2+1.5 -> option 0 (4 per block)
-    assert e["rho_kblock"] == 4
+    # rho: 1+1.15 / 1.33+1 / 2.2+1.5 -> option 0 (1 k vector per lane)
+    assert e["rho_kblock"] == 1
 
 
 def test_lookup_by_size(monkeypatch):
-    table = {"kinds": {"CVFF": {"mixed": {"kforce_lanes": 8, "rho_kblock": 4,
+    table = {"kinds": {"CVFF": {"mixed": {"kforce_lanes": 8, "rho_kblock": 2,
                                           "pair_lanes": [[5000, 16], [50000, 8], [2 ** 31 - 1, 4]]}}}}
     monkeypatch.setattr(tiles, "_cache", table)
-    assert tiles.choose("CVFF", "mixed", 3000) == {"kforce_lanes": 8, "rho_kblock": 4, "pair_lanes": 16}
+    assert tiles.choose("CVFF", "mixed", 3000) == {"kforce_lanes": 8, "rho_kblock": 2, "pair_lanes": 16}
     assert tiles.choose("CVFF", "mixed", 5001)["pair_lanes"] == 8
     assert tiles.choose("CVFF", "mixed", 10 ** 6)["pair_lanes"] == 4
     assert tiles.choose("CVFF", "fp64", 3000) == {}
 
 
 def test_params_explicit_tuned_default(monkeypatch):
-    table = {"kinds": {"LJ/CUT/COUL/LONG": {"fp64": {"kforce_lanes": 2, "rho_kblock": 16,
+    table = {"kinds": {"LJ/CUT/COUL/LONG": {"fp64": {"kforce_lanes": 2, "rho_kblock": 4,
                                                      "pair_lanes": [[2 ** 31 - 1, 8]]}}}}
     monkeypatch.setattr(tiles, "_cache", table)
     p = make_params(salt_config(), 4000)
-    assert (p.pair_lanes, p.kforce_lanes, p.rho_kblock) == (8, 2, 16)
+    assert (p.pair_lanes, p.kforce_lanes, p.rho_kblock) == (8, 2, 4)
     p = make_params(salt_config(pair_lanes=4), 4000)
-    assert (p.pair_lanes, p.kforce_lanes, p.rho_kblock) == (4, 2, 16)
+    assert (p.pair_lanes, p.kforce_lanes, p.rho_kblock) == (4, 2, 4)
     p = make_params(salt_config(tiles="default"), 4000)
     assert (p.pair_lanes, p.kforce_lanes, p.rho_kblock) == (0, 0, 0)
     p = make_params(salt_config())  # no atom count: library defaults
@@ -46,6 +46,6 @@ def test_committed_table_is_consistent():
     t = tiles.load()
     for ff, by_prec in t.get("kinds", {}).items():
         for prec, e in by_prec.items():
-            assert e["kforce_lanes"] in (2, 4, 8) and e["rho_kblock"] in (4, 8, 16)
+            assert e["kforce_lanes"] in (2, 4, 8) and e["rho_kblock"] in (1, 2, 4)
             edges = [edge for edge, _ in e["pair_lanes"]]
             assert edges == sorted(edges) and all(v in (4, 8, 16) for _, v in e["pair_lanes"])
