@@ -19,21 +19,20 @@
 
 namespace rbmd {
 
-// k-vectors per workgroup in the rho pass: a tile choice (4, 8 or 16, SysView::rho_kb)
-
 // e^{i k.r} with k = 2 pi m / L is e^{2 pi i t} with the phase t = m.(r/L) in
 // turns (several hundred radians at |m| ~ 30): sincos_turns reduces it
 // exactly to a quadrant and a remainder of at most 1/8 turn before its
 // polynomials, so no large-argument reduction is needed.  F32 (mixed
-// precision): the remainder's polynomials in FP32 (|error| <= 1.1e-7); the
-// phase, the reduction and every sum stay FP64.
+// precision): the phase reduced exactly in FP64 to at most 1/2 turn, rounded
+// to FP32, and the hardware sin / cos (sincos_turns32_hw, |error| <= 1.2e-7);
+// the phase, the reduction and every sum stay FP64.
 template <bool F32>
 __device__ __forceinline__ void phase_sincos(double mx, double my, double mz, double sx, double sy, double sz,
                                              double* sn, double* cs) {
   const double t = fma(mz, sz, fma(my, sy, mx * sx));
   if constexpr (F32) {
     float s32, c32;
-    sincos_turns32(t, &s32, &c32);
+    sincos_turns32_hw(t, &s32, &c32);
     *sn = s32;
     *cs = c32;
   } else {

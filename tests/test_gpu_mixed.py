@@ -1,8 +1,8 @@
 """Mixed precision (RBMD_PREC_MIXED) against the FP64 path on identical inputs.
 
-The mixed path evaluates the pair scalar (1/r, LJ, the erfc/exp table) and the
-sin/cos of every structure-factor and k-space force term in FP32 and sums in
-FP64.  Bar: forces within 1e-5 relative RMS of the FP64 path (RMS over atoms of
+The mixed path evaluates the pair scalar (1/r, LJ, the erfc/exp table) in FP32
+and the sin/cos of every structure-factor and k-space force term with the
+hardware's FP32 sin/cos of the FP64-reduced phase, and sums in FP64.  Bar: forces within 1e-5 relative RMS of the FP64 path (RMS over atoms of
 |dF_i| divided by the RMS of |F_i|) and the summed virial within 1e-5
 relative, on the same lists and the same k-vector sets.
 The FP64 path itself equals the oracle to 1e-10 (test_gpu_parity.py), so the
