@@ -1,9 +1,10 @@
 """Tile choices measured by rbmd_amd/tune.py (rbmd_amd/tiles.json).
 
-The table holds, per force field and precision, the lanes per atom of the
-k-space force kernel and the k vectors per structure-factor workgroup (one
-choice per kind and precision, so every system of a pack shares them) and
-the pair kernel's lanes per atom by atom count (a per-system runtime value).
+The table holds, per force field and precision, the wavefronts that split
+the k vectors of the k-space force kernel and the k vectors per lane of the
+structure-factor kernel (one choice per kind and precision, so every system
+of a pack shares them) and the pair kernel's lanes per atom by atom count (a
+per-system runtime value).
 Without a table the library's defaults apply.
 """
 import json

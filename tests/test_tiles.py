@@ -18,6 +18,20 @@ def test_choose_rule():
     assert e["rho_kblock"] == 1
 
 
+def test_choose_k_tiles_by_step_time():
+    """where step times are measured they decide the k-space tiles, not the kernel times"""
+    per_size = [
+        (1000, {"pair_lanes": [30, 20, 10], "kforce_lanes": [20, 10, 12], "rho_kblock": [9, 12, 20],
+                "step_kforce_lanes": [50, 52, 51], "step_rho_kblock": [60, 59, 58]}),
+        (100000, {"pair_lanes": [100, 90, 120], "kforce_lanes": [40, 39, 38], "rho_kblock": [30, 26, 40],
+                  "step_kforce_lanes": [300, 310, 320], "step_rho_kblock": [300, 290, 280]}),
+    ]
+    e = tune.choose(per_size)
+    assert e["kforce_lanes"] == 2  # by kernel time it would be 4
+    assert e["rho_kblock"] == 4  # by kernel time it would be 1
+    assert e["pair_lanes"] == [[10000, 16], [2 ** 31 - 1, 8]]
+
+
 def test_lookup_by_size(monkeypatch):
     table = {"kinds": {"CVFF": {"mixed": {"kforce_lanes": 8, "rho_kblock": 2,
                                           "pair_lanes": [[5000, 16], [50000, 8], [2 ** 31 - 1, 4]]}}}}
