@@ -402,10 +402,9 @@ void build_cells(rbmd_sys* s, bool bin_from_x) {
   c->run("cell_sort", [&] { launch_cell_sort(s->v, c->stream); });
 }
 
-// sorted copies of the new binning; the stored lists only on the list path
+// the stored lists (list path) on the sorted copies build_cells wrote
 void gather_and_list(rbmd_sys* s) {
   rbmd_ctx* c = s->ctx;
-  c->run("gather", [&] { launch_gather(s->v, c->stream); });
   s->locator_built = true;
   if (s->fused) {
     s->nbr_built = false;
@@ -1162,7 +1161,6 @@ rbmd_status rbmd_locator_geometry(const rbmd_sys* s, int32_t dims[3], int32_t* n
 rbmd_status rbmd_locator_build(rbmd_sys* s) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
   build_cells(s, true);
-  s->ctx->run("gather", [&] { launch_gather(s->v, s->ctx->stream); });
   s->locator_built = true;
   s->nbr_built = false;
   return check_launch("locator_build");
@@ -1525,16 +1523,18 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool the
   if (from_ring)  // the nhc phase above advanced the device step counter
     c->run("kvec_slot", [&] { launch_kvec_slot(s->kgraph, s->kring, s->ring_R, 3 * s->P, &s->nhc->step, st); });
   const double* scale = (tc == RBMD_TEMP_NOSE_HOOVER) ? &s->nhc->factor : nullptr;
-  c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, 1); });
-  build_cells(s, false);
+  c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, 1); });  // also bins
   if (shake) {
     c->run("shake", [&] { launch_shake(s->v, s->iv, st, s->seg_rows(SEG_SHAKE), s->shake_iter); });
     s->seg_used[SEG_SHAKE] = (s->nmol + kBlock - 1) / kBlock;
     if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 1);
   }
+  build_cells(s, false);  // the cell sort also writes the sorted copies: after SHAKE
   // the positions are final for this step: the k-space and bonded forces
   // branch off to the side stream while the list is built and the pair
-  // forces computed (kernel profiling keeps one stream, for its event timings)
+  // forces computed (kernel profiling keeps one stream, for its event timings).
+  // Forking before the cell sort instead put the k-space kernels beside the
+  // short cell kernels of the critical path and slowed the step.
   const bool fork = !c->prof && (has_k(s) || has_bonded(s));
   if (fork) {
     (void)hipEventRecord(s->ev_fork, st);
@@ -1757,7 +1757,7 @@ struct rbmd_pack {
 
 static const char* const kPackName[PK_COUNT] = {
     "pk_nhc",   "pk_kvec_slot", "pk_first_half", "pk_scan", "pk_scatter",     "pk_cell_sort",
-    "pk_one",   "pk_shake",     "pk_gather",     "pk_nbr_build", "pk_pair",   "pk_rho_partial",
+    "pk_one",   "pk_shake",     "pk_nbr_build", "pk_pair",   "pk_rho_partial",
     "pk_rho_reduce", "pk_kforce", "pk_bonded",   "pk_second_half", "pk_scale_v", "pk_rattle"};
 
 // the thermo-row descriptor a system's own step reduces (rows per producer =
@@ -1793,10 +1793,10 @@ static void pack_enqueue_step(rbmd_pack* p, bool thermo) {
   integ(PK_NHC, p->nh ? 1 : 5, 0, st);
   if (p->ring) ksp(PK_KSLOT, st);
   integ(PK_FIRST_HALF, p->tc == RBMD_TEMP_NOSE_HOOVER ? 1 : 0, p->lds_first, st);
+  if (p->shake) integ(PK_SHAKE, 0, 0, st);
   cells(PK_SCAN, 0);
   cells(PK_SCATTER, p->lds_scatter);
-  cells(PK_CELL_SORT, 0);
-  if (p->shake) integ(PK_SHAKE, 0, 0, st);
+  cells(PK_CELL_SORT, 0);  // also the sorted copies: after SHAKE
   // as in a single system's step: the k-space and bonded forces branch off to
   // the side stream once the positions are final (one stream when profiling)
   const bool fork = !c->prof && (p->has_k || p->has_b);
@@ -1812,7 +1812,6 @@ static void pack_enqueue_step(rbmd_pack* p, bool thermo) {
   }
   if (p->has_b) integ(PK_BONDED, 0, 0, sk);
   if (fork) (void)hipEventRecord(p->ev_join, sk);
-  cells(PK_GATHER, 0);
   cells(PK_NBR, 0);
   c->run(kPackName[PK_PAIR], [&] { launch_pack_pair(p->L[PK_PAIR], p->nblk[PK_PAIR], sig, st); });
   if (fork) (void)hipStreamWaitEvent(st, p->ev_join, 0);

@@ -147,7 +147,6 @@ void launch_bin(const SysView& s, hipStream_t st, const double* x, const double*
 void launch_scan(const SysView& s, hipStream_t st);
 void launch_scatter(const SysView& s, hipStream_t st);
 void launch_cell_sort(const SysView& s, hipStream_t st);
-void launch_gather(const SysView& s, hipStream_t st);
 void launch_nbr_build(const SysView& s, hipStream_t st);
 void launch_fill_int(int* p, int v, int n, hipStream_t st);
 void launch_fill_double(double* p, double v, int n, hipStream_t st);
@@ -281,7 +280,6 @@ enum PackKernel {
   PK_CELL_SORT,
   PK_ONE,  // one small workgroup per system (counter resets, status gather)
   PK_SHAKE,
-  PK_GATHER,
   PK_NBR,
   PK_PAIR,
   PK_RHO,

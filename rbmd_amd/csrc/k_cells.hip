@@ -170,7 +170,12 @@ void launch_scatter(const SysView& s, hipStream_t st) {
 
 constexpr int kSortCap = 4096;
 
-// one workgroup per cell: rank each id by the number of smaller ids in the cell
+// One workgroup per cell: rank each id by the number of smaller ids in the
+// cell, then write the atom's sorted copies at its sorted position p: x, y, z,
+// q, type, and the misfit flag (the current position lies in a different
+// cell than the binned one: possible after SHAKE moves atoms, or at x == hi,
+// where binning clamps and InWhichCell does not).  Runs after the step's
+// positions are final.
 __device__ __forceinline__ void cell_sort_body(const SysView& s, const int bid) {
   __shared__ int ids[kSortCap];
   int c = bid;
@@ -182,12 +187,25 @@ __device__ __forceinline__ void cell_sort_body(const SysView& s, const int bid) 
   }
   for (int t = threadIdx.x; t < cnt; t += blockDim.x) ids[t] = s.point_ids[start + t];
   __syncthreads();
+  const int a = c % s.dims[0];
+  const int b = (c / s.dims[0]) % s.dims[1];
+  const int e = c / (s.dims[0] * s.dims[1]);
   for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-    int me = ids[t];
+    const int me = ids[t];
     int r = 0;
     for (int k = 0; k < cnt; ++k) r += ids[k] < me;
-    s.point_ids[start + r] = me;
-    s.rank[me] = start + r;
+    const int p = start + r;
+    s.point_ids[p] = me;
+    s.rank[me] = p;
+    const double px = s.x[me], py = s.y[me], pz = s.z[me];
+    s.spos[p] = make_double4(px, py, pz, s.q[me]);
+    s.stype[p] = s.type[me];
+    const int ua = cell_axis(px, s.lo[0], s.dxdydz[0]);
+    const int ub = cell_axis(py, s.lo[1], s.dxdydz[1]);
+    const int uc = cell_axis(pz, s.lo[2], s.dxdydz[2]);
+    const int mf = (ua != a || ub != b || uc != e) ? 1 : 0;
+    s.misfit[p] = mf;
+    if (mf) s.misfit_list[atomicAdd(s.misfit_count, 1)] = p;
   }
 }
 
@@ -197,36 +215,6 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(SysView s) {
 
 void launch_cell_sort(const SysView& s, hipStream_t st) {
   hipLaunchKernelGGL(k_cell_sort, dim3(s.ncell), dim3(kBlock), 0, st, s);
-}
-
-// sorted copies of positions / charges / types; misfit = current position is in
-// a different cell than the binned one (possible after SHAKE moves atoms, or at
-// x == hi where binning clamps and InWhichCell does not)
-__device__ __forceinline__ void gather_body(const SysView& s, const int bid) {
-  int p = bid * blockDim.x + threadIdx.x;
-  if (p >= s.n) return;
-  int id = s.point_ids[p];
-  double px = s.x[id], py = s.y[id], pz = s.z[id];
-  s.spos[p] = make_double4(px, py, pz, s.q[id]);
-  s.stype[p] = s.type[id];
-  int cell = s.cell_of[id];
-  int a = cell % s.dims[0];
-  int b = (cell / s.dims[0]) % s.dims[1];
-  int c = cell / (s.dims[0] * s.dims[1]);
-  int ua = cell_axis(px, s.lo[0], s.dxdydz[0]);
-  int ub = cell_axis(py, s.lo[1], s.dxdydz[1]);
-  int uc = cell_axis(pz, s.lo[2], s.dxdydz[2]);
-  const int mf = (ua != a || ub != b || uc != c) ? 1 : 0;
-  s.misfit[p] = mf;
-  if (mf) s.misfit_list[atomicAdd(s.misfit_count, 1)] = p;
-}
-
-__global__ __launch_bounds__(kBlock) void k_gather(SysView s) {
-  gather_body(s, (int)blockIdx.x);
-}
-
-void launch_gather(const SysView& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_gather, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
 }
 
 // VERLET list build.  Three workgroups per work item (64 i-atoms of one
@@ -599,11 +587,6 @@ __global__ __launch_bounds__(kBlock) void kp_cell_sort(PACK_PARAMS) {
   cell_sort_body(e.v, bid);
 }
 
-__global__ __launch_bounds__(kBlock) void kp_gather(PACK_PARAMS) {
-  PACK_ENTRY(e, bid)
-  gather_body(e.v, bid);
-}
-
 __global__ __launch_bounds__(kNbrWaves * kWave) void kp_nbr_build(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
   nbr_build_verlet_body(e.v, bid);
@@ -621,9 +604,6 @@ void launch_pack_cells(int kernel, const PackLaunch& L, int nblocks, size_t lds,
       break;
     case PK_CELL_SORT:
       hipLaunchKernelGGL(kp_cell_sort, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
-      break;
-    case PK_GATHER:
-      hipLaunchKernelGGL(kp_gather, grid, dim3(kBlock), 0, st, PACK_ARGS(L));
       break;
     case PK_NBR:
       hipLaunchKernelGGL(kp_nbr_build, grid, dim3(kNbrWaves * kWave), 0, st, PACK_ARGS(L));

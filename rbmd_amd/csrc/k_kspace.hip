@@ -418,7 +418,6 @@ int pack_blocks(const SysView& s, const IntView& iv, int P, int kernel) {
       return (3 * P + kBlock - 1) / kBlock;
     case PK_FIRST_HALF:
     case PK_SCATTER:
-    case PK_GATHER:
     case PK_BONDED:
     case PK_SECOND:
     case PK_ATOMS:
