@@ -397,8 +397,12 @@ rbmd_status status_to_rc(int st) {
 void build_cells(rbmd_sys* s, bool bin_from_x) {
   rbmd_ctx* c = s->ctx;
   if (bin_from_x) c->run("bin", [&] { launch_bin(s->v, c->stream, s->v.x, s->v.y, s->v.z); });
-  c->run("scan", [&] { launch_scan(s->v, c->stream); });
-  c->run("scatter", [&] { launch_scatter(s->v, c->stream); });
+  if (scan_in_scatter(s->v)) {
+    c->run("scan_scatter", [&] { launch_scan_scatter(s->v, c->stream); });
+  } else {
+    c->run("scan", [&] { launch_scan(s->v, c->stream); });
+    c->run("scatter", [&] { launch_scatter(s->v, c->stream); });
+  }
   c->run("cell_sort", [&] { launch_cell_sort(s->v, c->stream); });
 }
 
@@ -1016,6 +1020,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   }
   hipStream_t st = ctx->stream;
   launch_fill_int(v.cell_count, 0, v.ncell, st);
+  launch_fill_int(v.cursor, 0, v.ncell, st);  // the one-launch scan + scatter counts from zero
   launch_fill_int(v.status, 0, 1, st);
   launch_fill_int(v.img, 0, 3 * n, st);
   launch_fill_double(s->rows, 0.0, (int)s->rows_len, st);

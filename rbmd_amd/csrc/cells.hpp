@@ -8,6 +8,7 @@ namespace rbmd {
 
 constexpr int kMisfitBlocks = 64;  // workgroups that take the misfit atoms
 constexpr int kColWaves = 9;        // stencil columns of the VERLET list build
+constexpr int kScanScatterCells = 2048;  // most cells for the one-launch scan + scatter
 constexpr int kNbrWaves = 3;        // columns (wavefronts) per list-build workgroup
 constexpr int kNbrParts = kColWaves / kNbrWaves;  // list-build workgroups per work item
 

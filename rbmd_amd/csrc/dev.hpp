@@ -147,6 +147,8 @@ void launch_bin(const SysView& s, hipStream_t st, const double* x, const double*
 void launch_scan(const SysView& s, hipStream_t st);
 void launch_scatter(const SysView& s, hipStream_t st);
 void launch_cell_sort(const SysView& s, hipStream_t st);
+bool scan_in_scatter(const SysView& s);  // few cells: the scan runs inside the scatter launch
+void launch_scan_scatter(const SysView& s, hipStream_t st);
 void launch_nbr_build(const SysView& s, hipStream_t st);
 void launch_fill_int(int* p, int v, int n, hipStream_t st);
 void launch_fill_double(double* p, double v, int n, hipStream_t st);

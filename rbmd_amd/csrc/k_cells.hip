@@ -93,37 +93,51 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds_wave, int* t
   return ex;
 }
 
-// one workgroup: cell_start = exclusive scan of counts; cursor = cell_start;
-// work items (cell, 64-atom chunk); counts reset to zero for the next binning
-__device__ __forceinline__ void scan_body(const SysView& s, const int bid) {
+// Exclusive scan of the cell counts by one workgroup.  With OUT it writes
+// cell_start, the work items (cell, 64-atom chunk), the misfit count and the
+// item count; with KERNEL (the scan kernel) also cursor = cell_start and the
+// counts reset to zero for the next binning.  start (LDS, or null) receives
+// every cell's start.
+template <bool KERNEL>
+__device__ __forceinline__ void cell_scan(const SysView& s, int* start, const bool out) {
   __shared__ int lds[17];
   int carry = 0, icarry = 0;
   for (int base = 0; base < s.ncell; base += blockDim.x) {
     int c = base + threadIdx.x;
     int cnt = c < s.ncell ? s.cell_count[c] : 0;
-    int nch = (cnt + kChunk - 1) / kChunk;
-    int tot, itot;
+    int tot;
     int ex = block_exclusive_scan(cnt, lds, &tot);
-    int iex = block_exclusive_scan(nch, lds, &itot);
-    if (c < s.ncell) {
-      s.cell_start[c] = carry + ex;
-      s.cursor[c] = carry + ex;
-      s.cell_count[c] = 0;
-      for (int k = 0; k < nch; ++k) {
-        int it = icarry + iex + k;
-        if (it < s.n_items) s.items[it] = (c << 8) | (k & 255);
+    if (start && c < s.ncell) start[c] = carry + ex;
+    if (out) {  // workgroup-uniform
+      int nch = (cnt + kChunk - 1) / kChunk;
+      int itot;
+      int iex = block_exclusive_scan(nch, lds, &itot);
+      if (c < s.ncell) {
+        s.cell_start[c] = carry + ex;
+        if (KERNEL) {
+          s.cursor[c] = carry + ex;
+          s.cell_count[c] = 0;
+        }
+        for (int k = 0; k < nch; ++k) {
+          int it = icarry + iex + k;
+          if (it < s.n_items) s.items[it] = (c << 8) | (k & 255);
+        }
+        if (nch > 256) atomicOr(s.status, 4);
       }
-      if (nch > 256) atomicOr(s.status, 4);
+      icarry += itot;
     }
     carry += tot;
-    icarry += itot;
   }
-  if (threadIdx.x == 0) {
+  if (out && threadIdx.x == 0) {
     s.misfit_count[0] = 0;
     s.cell_start[s.ncell] = carry;
     s.n_items_dev[0] = icarry < s.n_items ? icarry : s.n_items;
     if (icarry > s.n_items) atomicOr(s.status, 4);
   }
+}
+
+__device__ __forceinline__ void scan_body(const SysView& s, const int bid) {
+  cell_scan<true>(s, nullptr, true);
 }
 
 __global__ __launch_bounds__(1024) void k_scan(SysView s) {
@@ -168,6 +182,39 @@ void launch_scatter(const SysView& s, hipStream_t st) {
   hipLaunchKernelGGL(k_scatter, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), lds, st, s);
 }
 
+// Scan and scatter in one launch, for few cells (scan_in_scatter): every
+// workgroup scans the cell counts itself (block 0 also writes the scan's
+// outputs) and reserves its range in each cell with one atomic on cursor,
+// which starts at zero here (the cell sort resets it and the counts).  The
+// order inside a cell is fixed afterwards by k_cell_sort, as with k_scatter.
+__global__ __launch_bounds__(kBlock) void k_scan_scatter(SysView s) {
+  extern __shared__ int lds[];
+  int* start = lds;
+  int* cnt = lds + s.ncell;
+  int* base = cnt + s.ncell;
+  const int bid = (int)blockIdx.x;
+  cell_scan<false>(s, start, bid == 0);
+  const int i = bid * blockDim.x + threadIdx.x;
+  const int cell = i < s.n ? s.cell_of[i] : -1;
+  for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) cnt[c] = 0;
+  __syncthreads();
+  const int local = cell >= 0 ? atomicAdd(&cnt[cell], 1) : 0;
+  __syncthreads();
+  for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) {
+    const int v = cnt[c];
+    base[c] = v ? start[c] + atomicAdd(&s.cursor[c], v) : 0;
+  }
+  __syncthreads();
+  if (cell >= 0) s.point_ids[base[cell] + local] = i;
+}
+
+bool scan_in_scatter(const SysView& s) { return s.lds_bins && s.ncell <= kScanScatterCells; }
+
+void launch_scan_scatter(const SysView& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan_scatter, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock),
+                     3 * (size_t)s.ncell * sizeof(int), st, s);
+}
+
 constexpr int kSortCap = 4096;
 
 // One workgroup per cell: rank each id by the number of smaller ids in the
@@ -181,6 +228,10 @@ __device__ __forceinline__ void cell_sort_body(const SysView& s, const int bid) 
   int c = bid;
   int start = s.cell_start[c];
   int cnt = s.cell_start[c + 1] - start;
+  if (threadIdx.x == 0) {  // read by the binning and scatter kernels before this one
+    s.cell_count[c] = 0;
+    s.cursor[c] = 0;
+  }
   if (cnt > kSortCap) {
     if (threadIdx.x == 0) atomicOr(s.status, 4);
     return;
