@@ -144,6 +144,16 @@ typedef struct rbmd_params {
   int32_t pair_lanes;
   int32_t kforce_lanes;
   int32_t rho_kblock;
+  /* SHAKE convergence tolerance on the Lagrange-multiplier change between
+   * iterations; 0 = the reference's 1e-5 (MolecularWorklet.h:2507-2508) */
+  double shake_tol;
+  /* Verlet-list skin for MD steps (VERLET lists path, both boxes edges >= 2 (rc + skin)):
+   * 0 = the reference's list rebuilt every step (ExecutionMD.cpp:878-957);
+   * > 0 = one list to rc + skin kept over steps and rebuilt on the device when
+   * an atom has moved more than skin / 2 since the last build.  The pair
+   * kernel applies the reference's test rc^2 - d^2 > 1e-4 to every entry,
+   * so the pairs are the same; only the summation order differs. */
+  double skin;
 } rbmd_params;
 
 /* Thermodynamic scalars after a step; per-atom averages as in Para
@@ -230,14 +240,20 @@ rbmd_status rbmd_rbe_force(rbmd_sys* sys, double* force, double* virial6);
 rbmd_status rbmd_ewald_force(rbmd_sys* sys, const int32_t kmax[3], double* force, double* virial6,
                              double* energy_total);
 
-/* ---- bonded terms (MolecularWorklet.h:58-397) ---- */
+/* ---- the RBL path's erfc step table: ExecStaticTable::Extract + TableGnearValue
+ *      (ExecStaticTable.h:96-135) over the reference's tabulated G at alpha = 0.1
+ *      (ERFTable.h).  Evaluates the device lookup for n distances (host arrays). ---- */
+rbmd_status rbmd_static_table_gnear(rbmd_ctx* ctx, int32_t n, const double* dis, double* g_out);
+
+/* ---- bonded terms (MolecularWorklet.h:58-397); for CVFF + RBL + special bonds also the
+ *      intramolecular Coulomb correction SpecialCoulGeneral (RunWorklet.cpp:2012-2084) ---- */
 rbmd_status rbmd_force_bonded(rbmd_sys* sys, double* force, double* e_bond_sum, double* e_angle_sum);
 
 /* ---- MD loop: ExecutionNVT::Init (ExecutionNVT.cpp:60-74) then Solve per step (:90-135).
  *      rbmd_md_init evaluates the first force; rbmd_md_step enqueues n steps (PreSolve+Solve
  *      each) on the context stream without host round trips; rbmd_md_run_graph runs the same
- *      steps through a captured step graph where that is built (currently it enqueues as
- *      rbmd_md_step does). ---- */
+ *      steps by replaying one captured step graph (captured on first use and again after
+ *      anything it captured changes), with results bit-identical to rbmd_md_step. ---- */
 rbmd_status rbmd_md_init(rbmd_sys* sys);
 rbmd_status rbmd_md_step(rbmd_sys* sys, int64_t n_steps);
 rbmd_status rbmd_md_run_graph(rbmd_sys* sys, int64_t n_steps);

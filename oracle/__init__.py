@@ -55,6 +55,15 @@ def set_threads(n: int) -> int:
     return lib().orc_set_threads(int(n))
 
 
+def static_table_gnear(dis):
+    """G of the RBL erfc step table at each distance (ExecStaticTable.h:96-135)."""
+    d = np.ascontiguousarray(dis, np.float64)
+    g = np.zeros_like(d)
+    rc = lib().orc_static_table_gnear(C.c_int32(len(d)), _p(d), _p(g))
+    assert rc == 0
+    return g
+
+
 def rbe_S(alpha, box):
     b = np.ascontiguousarray(box, np.float64)
     return lib().orc_rbe_S(alpha, _p(b))

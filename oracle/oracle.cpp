@@ -358,16 +358,30 @@ inline double gnear(double alpha, double dis) {
   return (1.0 - std::erf(erfcx)) / (dis * dis) + 2 * std::sqrt(alpha) * std::exp(expx) / (std::sqrt(kPi) * dis);
 }
 
-// ERF step table at alpha = 0.1 (ExecStaticTable.h:96-135): G evaluated at the
-// lower edge of the FP32 bucket selected by 3 exponent + 8 mantissa bits.
+// ERF step table at alpha = 0.1: the reference's tabulated G values
+// (framework/model/ERFTable.h erf_gnear_leq2_v / erf_gnear_geq2_v, extracted by
+// tests/golden/make_erf_table.py), indexed as ExecStaticTable::Extract and
+// TableGnearValue do (ExecStaticTable.h:96-135): the float bits of the distance
+// masked with 2^26 - 2^15 (3 exponent + 8 mantissa bits) and shifted by 15,
+// minus the first index of the table chosen by dis < 2.  The reference reads
+// out of bounds for dis below 0.0998 or at 12 and beyond (hazard Q3); here the
+// bucket is clamped to the table (the product rejects rc >= 12 in table mode).
+#define RBMD_ERF_STORAGE static const
+#include "erf_table.inc"
+#undef RBMD_ERF_STORAGE
 inline double gnear_table(double dis) {
-  float fd = (float)dis;
+  const float fd = (float)dis;
   uint32_t bits;
   std::memcpy(&bits, &fd, 4);
-  bits &= 0xFFFF8000u;  // keep sign, exponent, top 8 mantissa bits
-  float lo;
-  std::memcpy(&lo, &bits, 4);
-  return gnear(0.1, (double)lo);
+  const int idx = (int)((bits & ((1u << 26) - (1u << 15))) >> 15);
+  if (fd < 2.0f) {
+    int k = idx - RBMD_ERF_LEQ2_INDEX0;
+    k = k < 0 ? 0 : (k >= RBMD_ERF_LEQ2_N ? RBMD_ERF_LEQ2_N - 1 : k);
+    return (double)kErfGnearLeq2[k];
+  }
+  int k = idx - RBMD_ERF_GEQ2_INDEX0;
+  k = k < 0 ? 0 : (k >= RBMD_ERF_GEQ2_N ? RBMD_ERF_GEQ2_N - 1 : k);
+  return (double)kErfGnearGeq2[k];
 }
 
 // ExecForceFunction::ComputeLJForce, ExecForceFunction.h:412-435
@@ -568,6 +582,49 @@ void force_rbl(orc_sys* s) {
   for (int i = 0; i < n; ++i) {
     s->f_near[i] = corr[i] - mean;
     for (int k = 0; k < 6; ++k) s->vir_near[(size_t)6 * i + k] = cvir[(size_t)6 * i + k] - vmean[k];
+  }
+}
+
+// RBL energies: ExecutionMD::ComputeLJCoulEnergy (ExecutionMD.cpp:670-750)
+// lists every pair with rc^2 - d^2 > 1e-4 and d^2 > 1e-4 over the RBL
+// locator's stencil (ComputeNeighbours, RunWorklet.cpp:296-355) and sums the
+// analytic LJ + erfc energies of LJCoulEnergyVerletVirialWorklet (:3145-3221)
+// or, with special bonds, SpecialLJCoulEnergyVerletWeightVirialWorklet
+// (:3047-3143): per atom e_lj = sum w E_LJ / 2, e_coul = qqr2e sum (E_erfc - (1-w) q q / 2r).
+void rbl_energy(orc_sys* s) {
+  int n = s->n;
+  double rc = s->p.cut_off, rc2 = rc * rc, alpha = s->p.alpha, qqr2e = s->u.qqr2e;
+  bool weighted = s->has_special && s->p.special_bonds;
+  s->e_lj.assign(n, 0.0);
+  s->e_coul.assign(n, 0.0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    V3 pi = s->x[i];
+    int ti = s->type[i];
+    double ei = s->eps[ti], si = s->sig[ti], qi = s->q[i];
+    double pelj = 0, pecoul = 0;
+    visit_stencil(s, i, [&](int j, V3 off, double d2) {
+      if (!(rc2 - d2 > 0.0001 && d2 > 0.0001)) return;
+      V3 r = min_image(s, s->x[j] - off, pi);
+      int tj = s->type[j];
+      double elj = lj_energy(r, ei, s->eps[tj], si, s->sig[tj], rc);
+      double dd2 = r.x * r.x + r.y * r.y + r.z * r.z;
+      double dis = std::sqrt(dd2);
+      bool in = dd2 < rc2 && dd2 > 0.0001;
+      double qj = s->q[j];
+      double ec = in ? 0.5 * qi * qj * (1.0 - std::erf(std::sqrt(alpha) * dis)) / dis : 0.0;
+      if (weighted) {
+        double w = special_weight(s, i, j);
+        double ecf = in ? 0.5 * qi * qj / dis : 0.0;
+        pelj += w * elj;
+        pecoul += qqr2e * (ec - (1 - w) * ecf);
+      } else {
+        pelj += elj;
+        pecoul += qqr2e * ec;
+      }
+    });
+    s->e_lj[i] = pelj;
+    s->e_coul[i] = pecoul;
   }
 }
 
@@ -1050,7 +1107,7 @@ void constraint_a(orc_sys* s) {
     double q3_0101 = im0 * im0 * r01sq, q3_1212 = im2 * im2 * r12sq, q3_2020 = (im0 + im2) * (im0 + im2) * r02sq;
     double q3_0120 = -2.0 * (im0 + im2) * im0 * r0120, q3_0112 = 2.0 * im0 * im2 * r0112;
     double q3_2012 = -2.0 * (im0 + im2) * im2 * r2012;
-    double tol = 0.00001;
+    double tol = s->p.shake_tol > 0 ? s->p.shake_tol : 0.00001;  // MolecularWorklet.h:2507-2508
     int max_iter = 5000;
     double l01 = 0, l20 = 0, l12 = 0;
     int niter = 0, done = 0, ovf = 0;
@@ -1187,12 +1244,14 @@ void compute_force(orc_sys* s) {
   // near force (ComputeVerletlistNearForce / ComputeRBLNearForce)
   if (s->cap == 0) s->cap = default_cap(s);
   if (build_neighbors(s, s->cap) != RBMD_OK) s->status_flags |= 4;
-  if (s->p.neighbor_type == RBMD_NEIGHBOR_RBL && ff != RBMD_FF_LJ_CUT)
+  if (s->p.neighbor_type == RBMD_NEIGHBOR_RBL) {
     force_rbl(s);
-  else
+    rbl_energy(s);  // ComputeRBLNearForce ends with ComputeLJCoulEnergy (ExecutionMD.cpp:670)
+  } else {
     force_verlet(s);
+  }
   double elj = 0, ecoul = 0;
-  if (s->p.neighbor_type != RBMD_NEIGHBOR_RBL || ff == RBMD_FF_LJ_CUT) {
+  {
     KSum a, b;
     for (int i = 0; i < n; ++i) {
       a.add(s->e_lj[i]);
@@ -1297,8 +1356,18 @@ int orc_set_threads(int n) {
 #endif
 }
 
+int orc_static_table_gnear(int32_t n, const double* dis, double* g) {
+  if (n < 0 || (n > 0 && (!dis || !g))) return RBMD_ERR_ARG;
+  for (int i = 0; i < n; ++i) g[i] = gnear_table(dis[i]);
+  return RBMD_OK;
+}
+
 int orc_create(const rbmd_sys_desc* d, const rbmd_params* p, orc_sys** out) {
   if (!d || !p || !out || d->n_atoms <= 0 || d->n_types <= 0) return RBMD_ERR_ARG;
+  // LJ/CUT over random-batch lists is the reference's separate LJForceRBL path
+  // (ExecutionMD::ComputeRBLLJForce, ExecutionMD.cpp:752-830; ComputeLJForceRBLWorklet,
+  // RunWorklet.cpp:1817-1910), not built here
+  if (p->neighbor_type == RBMD_NEIGHBOR_RBL && p->force_field == RBMD_FF_LJ_CUT) return RBMD_ERR_UNSUPPORTED;
   orc_sys* s = new orc_sys();
   s->p = *p;
   s->u = units_of(p->unit);
@@ -1447,10 +1516,9 @@ int orc_bruteforce_counts(const orc_sys* s, double rc, int32_t* num) {
 
 int orc_force_real(orc_sys* s, double* f, double* vir, double* elj, double* ecoul) {
   if (s->nb_num.empty()) return RBMD_ERR_STATE;
-  if (s->p.neighbor_type == RBMD_NEIGHBOR_RBL && s->p.force_field != RBMD_FF_LJ_CUT) {
+  if (s->p.neighbor_type == RBMD_NEIGHBOR_RBL) {
     force_rbl(s);
-    s->e_lj.assign(s->n, 0.0);
-    s->e_coul.assign(s->n, 0.0);
+    rbl_energy(s);
   } else {
     force_verlet(s);
   }
@@ -1542,8 +1610,13 @@ double orc_self_energy_total(const orc_sys* s) { return s->u.qqr2e * self_energy
 int orc_force_bonded(orc_sys* s, double* f, double* ebond_sum, double* eangle_sum) {
   double eb = bond_forces(s);
   double ea = angle_forces(s);
+  // CVFF + RBL + special bonds: the intramolecular Coulomb correction too, as
+  // ComputeAllForce adds it beside the bonded terms (ExecutionNVT.cpp:212-214)
+  const bool spec = s->p.force_field == RBMD_FF_CVFF && s->p.neighbor_type == RBMD_NEIGHBOR_RBL && s->p.special_bonds;
+  if (spec) force_special_coul(s);
   for (int i = 0; i < s->n; ++i) {
     V3 t = s->f_bond[i] + s->f_angle[i];
+    if (spec) t = s->f_spec[i] + t;
     if (f) {
       f[3 * i] = t.x;
       f[3 * i + 1] = t.y;

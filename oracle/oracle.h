@@ -54,7 +54,11 @@ int orc_neighbors_export(const orc_sys* s, int32_t cap, int32_t* ids, double* of
 /* brute-force O(N^2) all-image enumeration of pairs with 1e-4 < d^2 < rc^2 - 1e-4; counts per atom */
 int orc_bruteforce_counts(const orc_sys* s, double rc, int32_t* num);
 
-/* real-space force over the current neighbour lists (RunWorklet.cpp:2767-2985, 982-1266) */
+/* RBL erfc step table: ExecStaticTable::Extract + TableGnearValue (ExecStaticTable.h:96-135) */
+int orc_static_table_gnear(int32_t n, const double* dis, double* g);
+
+/* real-space force over the current neighbour lists (RunWorklet.cpp:2767-2985, 982-1266);
+ * RBL also forms the energies over a full rc list (ExecutionMD.cpp:670-750) */
 int orc_force_real(orc_sys* s, double* f, double* vir, double* elj, double* ecoul);
 
 /* RBE (RBEPSample.h:40-177; RunWorklet.cpp:2086-2141, 2209-2291) */

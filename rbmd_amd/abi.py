@@ -87,6 +87,8 @@ class Params(C.Structure):
         ("pair_lanes", C.c_int32),
         ("kforce_lanes", C.c_int32),
         ("rho_kblock", C.c_int32),
+        ("shake_tol", C.c_double),
+        ("skin", C.c_double),
     ]
 
 
@@ -214,4 +216,6 @@ def make_params(run, n_atoms=None):
     for k in tiles:
         v = int(getattr(run, k, 0) or 0)
         setattr(p, k, v if v else int(tiles[k]))
+    p.shake_tol = float(getattr(run, "shake_tol", 0.0) or 0.0)
+    p.skin = float(getattr(run, "skin", 0.0) or 0.0)
     return p

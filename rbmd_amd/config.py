@@ -48,6 +48,8 @@ class RunConfig:
     kforce_lanes: int = 0
     rho_kblock: int = 0
     thermo_interval: int = 0
+    shake_tol: float = 0.0  # 0: the reference's 1e-5 on the multiplier change
+    skin: float = 0.0  # Verlet-list skin of MD steps (0: a list per step, as the reference)
 
 
 def strip_json(text: str) -> str:
@@ -159,6 +161,8 @@ def run_config_from_json(doc: dict) -> RunConfig:
     rc.tiles = str(ext2.get("tiles", "tuned"))
     for key in ("pair_lanes", "kforce_lanes", "rho_kblock"):
         setattr(rc, key, int(ext2.get(key, 0)))
+    rc.shake_tol = float(ext2.get("shake_tol", 0.0))
+    rc.skin = float(ext2.get("skin", 0.0))
     return rc
 
 

@@ -437,7 +437,11 @@ void launch_real(rbmd_sys* s, int thermo, int peratom) {
   } else {
     s->v.nslab = 1;
     c->run("pair", [&] { launch_pair(s->v, c->stream, thermo, peratom, s->seg_rows(SEG_PAIR), &nblk); });
-    if (s->v.rbl) nblk = 0;  // the RBL kernels write no rows
+    // RBL: the force kernels write no rows; the energy pass (every step that
+    // forms thermo sums, as the reference forms them every step) writes the
+    // energies and the mean-removed RBL virial
+    if (s->v.rbl && (thermo || peratom))
+      c->run("rbl_energy", [&] { launch_rbl_energy(s->v, c->stream, peratom, s->seg_rows(SEG_PAIR), &nblk); });
   }
   if (thermo || peratom) s->seg_used[SEG_PAIR] = nblk;
 }
@@ -445,7 +449,9 @@ void launch_real(rbmd_sys* s, int thermo, int peratom) {
 bool has_k(const rbmd_sys* s) {
   return s->p.force_field != RBMD_FF_LJ_CUT && s->p.coulomb_type != RBMD_COULOMB_NONE;
 }
-bool has_bonded(const rbmd_sys* s) { return s->p.force_field == RBMD_FF_CVFF && (s->nb > 0 || s->na > 0); }
+bool has_bonded(const rbmd_sys* s) {
+  return s->p.force_field == RBMD_FF_CVFF && (s->nb > 0 || s->na > 0 || s->iv.spec_coul);
+}
 bool shake_on(const rbmd_sys* s) { return s->p.fix_shake == RBMD_SHAKE_TRUE && s->nmol > 0; }
 
 rbmd_status ensure_rho(rbmd_sys* s, int P) {
@@ -720,6 +726,15 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
       return fail(RBMD_ERR_UNSUPPORTED, "mixed precision is built for pair_path lists only");
     if (d->n_types > kMaxTypes) return fail(RBMD_ERR_UNSUPPORTED, "mixed precision needs n_types <= %d", kMaxTypes);
   }
+  if (p->neighbor_type == RBMD_NEIGHBOR_RBL && p->force_field == RBMD_FF_LJ_CUT)
+    return fail(RBMD_ERR_UNSUPPORTED,
+                "LJ/CUT over random-batch lists (the reference's LJForceRBL path, ExecutionMD.cpp:752-830) is not built");
+  if (p->neighbor_type == RBMD_NEIGHBOR_RBL && p->erfc_mode == RBMD_ERFC_TABLE && !(p->cut_off < 12.0))
+    return fail(RBMD_ERR_UNSUPPORTED, "the erfc step table covers r < 12 (ExecStaticTable.h:96-135); cut_off %g",
+                p->cut_off);
+  if (p->neighbor_type == RBMD_NEIGHBOR_RBL && !(p->r_core > 0 && p->r_core <= p->cut_off))
+    return fail(RBMD_ERR_ARG, "r_core must be in (0, cut_off]");
+  if (p->shake_tol < 0) return fail(RBMD_ERR_ARG, "shake_tol must be >= 0");
   for (int k = 0; k < 3; ++k)
     if (!(d->hi[k] > d->lo[k])) return fail(RBMD_ERR_ARG, "empty box along axis %d", k);
   for (int i = 0; i < d->n_atoms; ++i) {
@@ -877,6 +892,40 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   iv.mvv2e = s->u.mvv2e;
   iv.nb = s->nb;
   iv.na = s->na;
+  iv.shake_tol = p->shake_tol > 0 ? p->shake_tol : 0.00001;
+  // molecule groups of SpecialCoulGeneral: every atom lists the atoms of its
+  // molecule in ascending id (SetMolecularGroup, ModelFileInitCondition.cpp:698-736);
+  // the correction runs for CVFF over random-batch lists with special bonds
+  // (ExecutionNVT.cpp:212-214, SpecialCoulForce :673-702)
+  if (v.rbl && p->force_field == RBMD_FF_CVFF && p->special_bonds && d->mol_id) {
+    int nm = 0;
+    for (int i = 0; i < n; ++i) {
+      if (d->mol_id[i] < 0) {
+        rc = fail(RBMD_ERR_ARG, "atom %d has a negative molecule id", i);
+        break;
+      }
+      nm = std::max(nm, d->mol_id[i] + 1);
+    }
+    if (rc == RBMD_OK) {
+      std::vector<int> moff(nm + 1, 0);
+      for (int i = 0; i < n; ++i) moff[d->mol_id[i] + 1]++;
+      for (int m = 0; m < nm; ++m) moff[m + 1] += moff[m];
+      std::vector<int> members(n), cur(moff.begin(), moff.end() - 1);
+      for (int i = 0; i < n; ++i) members[cur[d->mol_id[i]]++] = i;  // ascending id within a molecule
+      std::vector<int> goff(n + 1, 0), gids;
+      for (int i = 0; i < n; ++i) {
+        const int m = d->mol_id[i];
+        gids.insert(gids.end(), members.begin() + moff[m], members.begin() + moff[m + 1]);
+        goff[i + 1] = (int)gids.size();
+      }
+      int* tmp = nullptr;
+      A(s->upload(&tmp, goff.data(), goff.size()));
+      iv.grp_off = tmp;
+      A(s->upload(&tmp, gids.data(), std::max<size_t>(gids.size(), 1)));
+      iv.grp_ids = tmp;
+      iv.spec_coul = 1;
+    }
+  }
   {
     std::vector<int> boff(n + 1, 0), aoff(n + 1, 0);
     for (int b = 0; b < s->nb; ++b) {
@@ -971,7 +1020,8 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   // reduction rows: one segment per producer, sized for its largest launch
   {
     const int nblk = (n + kBlock - 1) / kBlock;
-    const int cap[NSEG] = {std::max(pair_blocks(v), force_cells_blocks(v)), kforce_blocks(v), nblk, nblk,
+    const int pair_rows = std::max(std::max(pair_blocks(v), force_cells_blocks(v)), v.rbl ? rbl_energy_blocks(v) : 0);
+    const int cap[NSEG] = {pair_rows, kforce_blocks(v), nblk, nblk,
                            (std::max(s->nmol, 1) + kBlock - 1) / kBlock};
     size_t off = 0;
     for (int g = 0; g < NSEG; ++g) {
@@ -1308,8 +1358,8 @@ rbmd_status rbmd_force_real(rbmd_sys* s, double* force, double* vir, double* elj
     }
     if (vir)
       for (int k = 0; k < 6; ++k) vir[6 * a + k] = slab_sum(vv, i, 6, k);
-    if (elj) elj[a] = s->v.rbl ? 0.0 : slab_sum(el, i, 1, 0);
-    if (ecoul) ecoul[a] = s->v.rbl ? 0.0 : slab_sum(ec, i, 1, 0);
+    if (elj) elj[a] = slab_sum(el, i, 1, 0);
+    if (ecoul) ecoul[a] = slab_sum(ec, i, 1, 0);
   }
   int stt = 0;
   RC_TRY(read_status(s, &stt));
@@ -1441,6 +1491,26 @@ rbmd_status rbmd_ewald_force(rbmd_sys* s, const int32_t kmax[3], double* force, 
     RC_TRY(long_energy(s, kmax, &e_avg, &grid));
     *energy_total = s->u.qqr2e * grid * (2 * kPi / s->v.volume);
   }
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_static_table_gnear(rbmd_ctx* c, int32_t n, const double* dis, double* g) {
+  if (!c || n < 0 || (n > 0 && (!dis || !g))) return fail(RBMD_ERR_ARG, "bad arguments");
+  if (n == 0) return RBMD_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  double *dd = nullptr, *dg = nullptr;
+  HIP_TRY(hipMalloc(&dd, (size_t)n * 8));
+  hipError_t e = hipMalloc(&dg, (size_t)n * 8);
+  if (e == hipSuccess) e = hipMemcpyAsync(dd, dis, (size_t)n * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_static_table(dd, dg, n, c->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(g, dg, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dd);
+  if (dg) (void)hipFree(dg);
+  if (e != hipSuccess) return fail(RBMD_ERR_HIP, "static table: %s", hipGetErrorString(e));
   return RBMD_OK;
 }
 

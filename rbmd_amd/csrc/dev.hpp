@@ -157,6 +157,12 @@ void launch_fill_double(double* p, double v, int n, hipStream_t st);
 // partials: [nblocks][kRowPair] block sums (may be null when thermo == 0)
 void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, double* partials, int* nblocks);
 int pair_blocks(const SysView& s);
+// RBL energies over a full rc stencil (no list) and the pair thermo rows
+// (energies + the mean-removed RBL virial); e_lj/e_coul per atom when peratom
+void launch_rbl_energy(const SysView& s, hipStream_t st, int peratom, double* rows, int* nblocks);
+int rbl_energy_blocks(const SysView& s);
+// the RBL erfc step table at n distances (device arrays)
+void launch_static_table(const double* dis, double* g, int n, hipStream_t st);
 // fused cell-stencil real-space force (no stored list); fn* get 3 slab partials
 void launch_force_cells(const SysView& s, hipStream_t st, int thermo, int peratom, double* partials, int* nblocks);
 int force_cells_blocks(const SysView& s);
@@ -205,6 +211,11 @@ struct IntView {
   const double *angle_k, *angle_t0;
   const int *inc_b_off, *inc_b;   // atom -> bonds
   const int *inc_a_off, *inc_a;   // atom -> angles
+  double shake_tol;               // SHAKE convergence on the multiplier change
+  // SpecialCoulGeneral (CVFF + RBL + special bonds): atom -> the atoms of its
+  // molecule, itself included, in ascending id (CSR over atoms)
+  int spec_coul;
+  const int *grp_off, *grp_ids;
 };
 void launch_first_half(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin);
 void launch_shake(const SysView& s, const IntView& iv, hipStream_t st, double* partials, int* max_iter);

@@ -155,7 +155,7 @@ __device__ __forceinline__ void shake_body(const SysView& s, const IntView& iv, 
     double q3_0101 = im0 * im0 * r01sq, q3_1212 = im2 * im2 * r12sq, q3_2020 = (im0 + im2) * (im0 + im2) * r02sq;
     double q3_0120 = -2.0 * (im0 + im2) * im0 * r0120, q3_0112 = 2.0 * im0 * im2 * r0112;
     double q3_2012 = -2.0 * (im0 + im2) * im2 * r2012;
-    const double tol = 0.00001;
+    const double tol = iv.shake_tol;  // the reference's 0.00001 unless set (MolecularWorklet.h:2507-2508)
     const int max_iter = 5000;
     double l01 = 0, l20 = 0, l12 = 0;
     int niter = 0, done = 0, ovf = 0;
@@ -326,13 +326,50 @@ void launch_rattle(const SysView& s, const IntView& iv, hipStream_t st) {
 }
 
 // bond + angle forces gathered per atom over its incident terms; the energy
-// and virial of a term are counted by its first atom
+// and virial of a term are counted by its first atom.  With iv.spec_coul
+// (CVFF + RBL + special bonds) also the intramolecular Coulomb correction of
+// ComputeSpecialCoulGeneralWorklet (RunWorklet.cpp:2012-2084, added to the
+// force by ExecutionNVT::ComputeAllForce, ExecutionNVT.cpp:212-214): for the
+// other atoms j of i's molecule (molecules of 3 or more atoms),
+//   F_i += (1 - w_ij) (-332.06371 q_i q_j / r^3) r,  r = MinDistanceVec(x_i, x_j),
+// 332.06371 hard-coded as in the reference (hazard Q24), w_ij the last match of
+// j in i's special list (else 1); its virial +1/2 r (x) F (ComputePairVirial_fix)
+// goes into the bond virial slots (both enter the pressure once,
+// ExecutionMD.cpp:1323-1358).  The random-batch real space has no exclusions,
+// so this removes the intramolecular erfc Coulomb the pair term added.
 template <bool THERMO>
 __device__ __forceinline__ void bonded_body(const SysView& s, const IntView& iv, double* rows, const int bid) {
   int i = bid * blockDim.x + threadIdx.x;
   double f[3] = {0, 0, 0};
   double eb = 0, ea = 0;
   double vb[6] = {0, 0, 0, 0, 0, 0}, va[6] = {0, 0, 0, 0, 0, 0};
+  if (i < s.n && iv.spec_coul && iv.grp_off[i + 1] - iv.grp_off[i] >= 3) {
+    const D3 pi = ld(s.x, s.y, s.z, i);
+    const double qi = s.q[i];
+    for (int k = iv.grp_off[i]; k < iv.grp_off[i + 1]; ++k) {
+      const int j = iv.grp_ids[k];
+      if (j == i) continue;
+      const D3 r = mimg(s, pi, ld(s.x, s.y, s.z, j));
+      const double dis = sqrt(dotp(r, r));
+      const double fcomp = -332.06371 * qi * s.q[j] / (dis * dis * dis);
+      double w = 1.0;
+      if (s.sp_off)
+        for (int t = s.sp_off[i]; t < s.sp_off[i + 1]; ++t)
+          if (s.sp_ids[t] == j) w = s.sp_w[t];
+      const D3 cp = mul((1.0 - w) * fcomp, r);
+      f[0] += cp.x;
+      f[1] += cp.y;
+      f[2] += cp.z;
+      if (THERMO) {
+        vb[0] += 0.5 * (r.x * cp.x);
+        vb[1] += 0.5 * (r.y * cp.y);
+        vb[2] += 0.5 * (r.z * cp.z);
+        vb[3] += 0.5 * (r.x * cp.y);
+        vb[4] += 0.5 * (r.x * cp.z);
+        vb[5] += 0.5 * (r.y * cp.z);
+      }
+    }
+  }
   if (i < s.n) {
     for (int k = iv.inc_b_off[i]; k < iv.inc_b_off[i + 1]; ++k) {
       int b = iv.inc_b[k];

@@ -21,6 +21,7 @@
 #include "pack.hpp"
 #include "reduce.hpp"
 #include "pair.hpp"
+#include "cells.hpp"
 
 namespace rbmd {
 
@@ -392,6 +393,126 @@ __global__ __launch_bounds__(1024) void k_rbl_mean(SysView s) {
     if (vir)
       for (int k = 0; k < 6; ++k) s.vir_near[(size_t)6 * i + k] -= m[3 + k];
   }
+}
+
+// RBL energies.  The reference forms them every RBL step over a full Verlet
+// list to rc (ExecutionMD::ComputeLJCoulEnergy, ExecutionMD.cpp:670-750:
+// ComputeNeighbours over the RBL locator, then LJCoulEnergyVerletVirialWorklet
+// RunWorklet.cpp:3145-3221, or SpecialLJCoulEnergyVerletWeightVirialWorklet
+// :3047-3143 with special bonds): per pair with rc^2 - d^2 > 1e-4, d^2 > 1e-4
+// on the cell-offset vector, the minimum-image vector gives
+//   e_lj   += w * 4 eps (s6/d6 - 1) s6/d6 / 2            (ComputePotentialEn_fix)
+//   e_coul += qqr2e (q q erfc(sqrt(alpha) r) / 2r - (1 - w) q q / 2r)
+// with the analytic erfc at the configured alpha (not the step table).  No
+// list is stored: one wavefront per atom walks the (2nc+1)^3 stencil around
+// the cell its current position lies in (InWhichCell, so misfit atoms need no
+// separate path), lanes over each cell's atoms.  Runs on the steps that form
+// thermo sums; its rows also carry the mean-removed RBL virial of each atom.
+template <bool WEIGHTED, bool PERATOM>
+__global__ __launch_bounds__(kBlock) void k_rbl_energy(SysView s, double* rows) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & (kWave - 1);
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave);
+  double elj = 0.0, ec = 0.0;
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  if (i < s.n) {
+    const double4 pi = s.spos[i];
+    const int ti = s.stype[i];
+    int spj[kMaxSpecial];
+    double spw[kMaxSpecial];
+    if (WEIGHTED) {
+      const int id = s.point_ids[i];
+      const int b = s.sp_off[id];
+      const int spn = min(s.sp_off[id + 1] - b, kMaxSpecial);  // longer lists are refused at creation
+#pragma unroll
+      for (int k = 0; k < kMaxSpecial; ++k) {
+        spj[k] = k < spn ? s.rank[s.sp_ids[b + k]] : -1;
+        spw[k] = k < spn ? s.sp_w[b + k] : 1.0;
+      }
+    }
+    const int ca = cell_axis(pi.x, s.lo[0], s.dxdydz[0]);
+    const int cb = cell_axis(pi.y, s.lo[1], s.dxdydz[1]);
+    const int cc = cell_axis(pi.z, s.lo[2], s.dxdydz[2]);
+    const int nc = s.nc;
+    for (int a = -nc; a <= nc; ++a)
+      for (int b = -nc; b <= nc; ++b)
+        for (int e = -nc; e <= nc; ++e) {
+          const StencilCell t = stencil_cell(s, ca, cb, cc, a, b, e);
+          if (!t.ok) continue;  // an atom more than a box away: flagged by the force pass
+          for (int base = t.js; base < t.je; base += kWave) {
+            const int j = base + lane;
+            if (j >= t.je) continue;
+            const double4 pj = s.spos[j];
+            double rx = (pj.x - t.ox) - pi.x;
+            double ry = (pj.y - t.oy) - pi.y;
+            double rz = (pj.z - t.oz) - pi.z;
+            double d2 = dist2(rx, ry, rz);
+            if (!(s.cut2 - d2 > 0.0001 && d2 > 0.0001)) continue;
+            min_image(s, rx, ry, rz);  // MinDistanceVec(p_j, p_i)
+            d2 = dist2(rx, ry, rz);
+            if (!(d2 < s.cut2 && d2 > 0.0001)) continue;
+            const int tix = ti * s.ntypes + s.stype[j];
+            const double eps = s.eps_ij[tix], s6 = s.sig6_ij[tix];
+            const double d6 = d2 * d2 * d2;
+            const double el = 0.5 * (4.0 * eps * (s6 / d6 - 1.0) * (s6 / d6));
+            const double dis = sqrt(d2);
+            const double qq = pi.w * pj.w;
+            const double eco = 0.5 * qq * erfc(s.sqrt_alpha * dis) / dis;
+            double w = 1.0;
+            if (WEIGHTED) {
+#pragma unroll
+              for (int u = 0; u < kMaxSpecial; ++u)
+                if (spj[u] == j) w = spw[u];
+              elj += w * el;
+              ec += s.qqr2e * (eco - (1.0 - w) * (0.5 * qq / dis));
+            } else {
+              elj += el;
+              ec += s.qqr2e * eco;
+            }
+          }
+        }
+    elj = wave_sum(elj);
+    ec = wave_sum(ec);
+    if (lane == 0) {
+      if (PERATOM) {
+        s.e_lj[i] = elj;
+        s.e_coul[i] = ec;
+      }
+      for (int k = 0; k < 6; ++k) vir[k] = s.vir_near[(size_t)6 * i + k];
+    } else {
+      elj = ec = 0.0;
+    }
+  }
+  const double vals[kRowPair] = {elj, ec, vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
+  block_write_row<kRowPair>(vals, rows + (size_t)blockIdx.x * kRowPair);
+}
+
+int rbl_energy_blocks(const SysView& s) { return (s.n + kBlock / kWave - 1) / (kBlock / kWave); }
+
+void launch_rbl_energy(const SysView& s, hipStream_t st, int peratom, double* rows, int* nblocks) {
+  const int grid = rbl_energy_blocks(s);
+  if (nblocks) *nblocks = grid;
+  if (s.weighted) {
+    if (peratom)
+      hipLaunchKernelGGL((k_rbl_energy<true, true>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+    else
+      hipLaunchKernelGGL((k_rbl_energy<true, false>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+  } else {
+    if (peratom)
+      hipLaunchKernelGGL((k_rbl_energy<false, true>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+    else
+      hipLaunchKernelGGL((k_rbl_energy<false, false>), dim3(grid), dim3(kBlock), 0, st, s, rows);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_static_table(const double* dis, double* g, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) g[i] = gnear_table(dis[i]);
+}
+
+void launch_static_table(const double* dis, double* g, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_static_table, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, dis, g, n);
 }
 
 int pair_blocks(const SysView& s) {

@@ -9,13 +9,26 @@ __device__ __forceinline__ double gnear(double alpha, double sqrt_alpha, double 
   return erfc(sqrt_alpha * dis) / (dis * dis) + 2.0 * sqrt_alpha * exp(-alpha * dis * dis) / (sqrt(kPi) * dis);
 }
 
-// step table of the RBL path: G at alpha = 0.1 evaluated at the lower edge of
-// the FP32 bucket given by 3 exponent + 8 mantissa bits (ExecStaticTable.h:38-39,127-135)
+// Step table of the RBL path: the reference's tabulated G at alpha = 0.1
+// (ERFTable.h; erf_table.inc is generated from it), looked up as
+// ExecStaticTable::Extract + TableGnearValue do (ExecStaticTable.h:38-39,
+// 96-135): bucket = (float bits & (2^26 - 2^15)) >> 15, minus the first index
+// of the table chosen by dis < 2.  The reference reads out of bounds below
+// dis = 0.0998 and from 12 on (hazard Q3): here the bucket is clamped to the
+// table, and table mode with rc >= 12 is refused when the system is created.
+#define RBMD_ERF_STORAGE static __device__ const
+#include "erf_table.inc"
+#undef RBMD_ERF_STORAGE
+
 __device__ __forceinline__ double gnear_table(double dis) {
-  float fd = (float)dis;
-  unsigned bits = __float_as_uint(fd) & 0xFFFF8000u;
-  double lo = (double)__uint_as_float(bits);
-  return gnear(0.1, sqrt(0.1), lo);
+  const float fd = (float)dis;
+  const int idx = (int)((__float_as_uint(fd) & ((1u << 26) - (1u << 15))) >> 15);
+  if (fd < 2.0f) {
+    const int k = min(max(idx - RBMD_ERF_LEQ2_INDEX0, 0), RBMD_ERF_LEQ2_N - 1);
+    return (double)kErfGnearLeq2[k];
+  }
+  const int k = min(max(idx - RBMD_ERF_GEQ2_INDEX0, 0), RBMD_ERF_GEQ2_N - 1);
+  return (double)kErfGnearGeq2[k];
 }
 
 __device__ __forceinline__ void min_image(const SysView& s, double& rx, double& ry, double& rz) {

@@ -26,7 +26,7 @@ EXPORTS = [
     "rbmd_rbe_structure_factor", "rbmd_rbe_force", "rbmd_ewald_force", "rbmd_force_bonded", "rbmd_md_init",
     "rbmd_md_step", "rbmd_md_run_graph", "rbmd_md_enqueue", "rbmd_md_wait", "rbmd_md_query", "rbmd_md_thermo", "rbmd_md_energy", "rbmd_profile_enable",
     "rbmd_profile_reset", "rbmd_profile_get", "rbmd_abi_sizeof", "rbmd_pack_create", "rbmd_pack_destroy",
-    "rbmd_pack_step", "rbmd_pack_enqueue", "rbmd_pack_wait",
+    "rbmd_pack_step", "rbmd_pack_enqueue", "rbmd_pack_wait", "rbmd_static_table_gnear",
 ]
 
 
@@ -89,6 +89,7 @@ def load():
     L.rbmd_pack_step.argtypes = [vp, C.c_int64]
     L.rbmd_pack_enqueue.argtypes = [vp, C.c_int64]
     L.rbmd_pack_wait.argtypes = [vp, pi]
+    L.rbmd_static_table_gnear.argtypes = [vp, C.c_int32, pd, pd]
     L.rbmd_abi_sizeof.argtypes = [C.c_int32]
     L.rbmd_abi_sizeof.restype = C.c_int32
     for name in EXPORTS:
@@ -124,6 +125,14 @@ def rbe_sample_host(alpha, box, p, seed, stream):
     m = np.zeros(3 * p)
     check(load().rbmd_rbe_sample_host(alpha, _p(b), int(p), int(seed), int(stream), _p(m)))
     return m.reshape(p, 3)
+
+
+def static_table_gnear(ctx, dis):
+    """the RBL erfc step table looked up on the device (ExecStaticTable.h:96-135)"""
+    d = np.ascontiguousarray(dis, np.float64)
+    g = np.zeros_like(d)
+    check(load().rbmd_static_table_gnear(ctx.h, C.c_int32(len(d)), _p(d), _p(g)))
+    return g
 
 
 class Context:
