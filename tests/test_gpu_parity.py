@@ -224,6 +224,24 @@ def test_md_water_nve(ctx):
     _md_compare(ctx, water(), water_run(temp_ctrl="NONE"), 10, 1e-8)
 
 
+def test_md_salt_temp_rescale(ctx):
+    _md_compare(ctx, salt(n=1000, seed=4), salt_run(temp_ctrl="TEMP_RESCALE", temperature=[1.2, 1.2, 0.1]), 10, 1e-8)
+
+
+def test_ewald_force_disordered_salt(ctx):
+    """per-atom converged-Ewald k-space force and virial on a disordered box
+    (the reference every RBE-vs-Ewald error is measured against)"""
+    s = salt(n=2000, seed=17)
+    run = salt_run(coulomb_type="EWALD", kmax=[12, 12, 12])
+    g, o = pair(ctx, s, run)
+    gf, gv, ge = g.ewald_force([12, 12, 12])
+    of, ov, oe = o.ewald_force([12, 12, 12])
+    assert np.abs(of).max() > 1.0  # not a lattice: the forces do not vanish
+    assert rel_err(gf, of) < FP64_TOL
+    assert rel_err(gv, ov) < 1e-9
+    assert abs(ge - oe) <= FP64_TOL * abs(oe)
+
+
 def test_rbl_lists_and_force(ctx):
     s = salt(n=4000, seed=3)
     r = salt_run(neighbor_type="RBL", r_core=2.0, neighbor_sample_num=40, erfc_mode="analytic")
