@@ -175,7 +175,7 @@ typedef struct rbmd_thermo {
   int32_t shake_max_iter;         /* most SHAKE iterations of any molecule this step */
   int32_t status_flags;           /* bit0 nonfinite, bit1 shake not converged, bit2 capacity */
   int32_t misfit_atoms;           /* atoms listed from their own stencil in the last list build */
-  int32_t reserved;
+  int32_t list_builds;            /* cell lists sorted (neighbour lists built) since creation */
 } rbmd_thermo;
 
 typedef struct rbmd_ctx rbmd_ctx;

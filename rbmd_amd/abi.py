@@ -110,7 +110,7 @@ class Thermo(C.Structure):
         ("shake_max_iter", C.c_int32),
         ("status_flags", C.c_int32),
         ("misfit_atoms", C.c_int32),
-        ("reserved", C.c_int32),
+        ("list_builds", C.c_int32),
     ]
 
     def as_dict(self):

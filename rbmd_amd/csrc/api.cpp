@@ -190,6 +190,18 @@ struct rbmd_sys {
   KRec* krec = nullptr;  // [P] k records of the force pass
   double* rho = nullptr;
   int rho_part_cap = 0;  // doubles
+  // cell geometry of the reference's per-step list (cells of width >= rc) and,
+  // for a skin list, of the MD step's list (cells of width >= rc + skin)
+  struct Geom {
+    int dims[3];
+    double dxdydz[3];
+    int ncell, n_items, lds_bins;
+    double list_cut2;
+  };
+  Geom g_ref{}, g_skin{};
+  bool skin = false;        // MD steps keep a list with a skin (params.skin > 0)
+  bool skin_active = false; // v currently holds the skin geometry
+  bool skin_ready = false;  // the skin list's buffers and build request are in place for the next step
   // host mirrors
   bool locator_built = false, nbr_built = false, initialized = false;
   double e_long = 0;
@@ -331,11 +343,11 @@ double column_fraction(double wx, double wy, double rc) {
 }
 
 // Default list capacity.  VERLET: per column segment, 1.15 x the mean count
-// in the largest column share of the cutoff sphere + 64; RBL: the core region
-// plus twice the mean shell sample + 64.
+// in the largest column share of the list's cutoff sphere (rc, or rc + skin)
+// + 64; RBL: the core region plus twice the mean shell sample + 64.
 int default_cap(const rbmd_sys* s) {
   double rho = s->n / s->v.volume;
-  double rc = s->p.cut_off;
+  double rc = s->v.rbl ? s->p.cut_off : std::sqrt(s->v.list_cut2);
   double c = rho * 4.0 / 3.0 * kPi * rc * rc * rc;
   int st = (2 * s->v.nc + 1);
   long maxc = (long)std::max(s->n, 1) * st * st * st;
@@ -377,6 +389,41 @@ rbmd_status ensure_nbr(rbmd_sys* s, int cap) {
   s->v.cap = nseg * seg_cap;
   s->graph_ok = false;
   return s->dalloc(&s->v.nbr, (size_t)s->v.cap * s->n);
+}
+
+// Put the reference geometry (per-step list) or the skin geometry into the
+// system's view.  The cell and list buffers are shared: switching drops the
+// built cells and lists, and a skin list is rebuilt on the next MD step.
+rbmd_status use_geom(rbmd_sys* s, bool skin) {
+  if (s->skin_active == skin) return RBMD_OK;
+  const rbmd_sys::Geom& g = skin ? s->g_skin : s->g_ref;
+  for (int d = 0; d < 3; ++d) {
+    s->v.dims[d] = g.dims[d];
+    s->v.dxdydz[d] = g.dxdydz[d];
+  }
+  s->v.ncell = g.ncell;
+  s->v.n_items = g.n_items;
+  s->v.lds_bins = g.lds_bins;
+  s->v.list_cut2 = g.list_cut2;
+  s->v.skin_on = skin ? 1 : 0;
+  s->skin_active = skin;
+  s->skin_ready = false;
+  s->locator_built = s->nbr_built = false;
+  s->graph_ok = false;
+  // the counters the cell kernels expect at zero (launch_fill: not a memset node)
+  launch_fill_int(s->v.cell_count, 0, std::max(s->g_ref.ncell, s->g_skin.ncell), s->ctx->stream);
+  launch_fill_int(s->v.cursor, 0, std::max(s->g_ref.ncell, s->g_skin.ncell), s->ctx->stream);
+  return ensure_nbr(s, 0);
+}
+
+// Before MD steps of a skin-list system: the skin geometry, and a rebuild
+// requested for the next step when the list is not current.
+rbmd_status prepare_skin(rbmd_sys* s) {
+  if (!s->skin || s->skin_ready) return RBMD_OK;
+  RC_TRY(use_geom(s, true));
+  launch_skin_request(s->v, 1, s->ctx->stream);
+  s->skin_ready = true;
+  return RBMD_OK;
 }
 
 rbmd_status read_status(rbmd_sys* s, int* st) {
@@ -802,6 +849,39 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   if (v.rbl) rbl_capacity(*p, n, v.volume, &v.rs_num, &v.pice_num);
   v.n_items = v.ncell + n / kChunk + 1;
   v.lds_bins = v.ncell <= 4096 ? 1 : 0;
+  v.list_cut2 = v.cut2;
+  s->g_ref = rbmd_sys::Geom{{v.dims[0], v.dims[1], v.dims[2]}, {v.dxdydz[0], v.dxdydz[1], v.dxdydz[2]},
+                            v.ncell, v.n_items, v.lds_bins, v.cut2};
+  s->g_skin = s->g_ref;
+  if (p->skin > 0) {
+    // a list to rc + skin over cells at least that wide; every box edge at least
+    // 2 (rc + skin), so no pair vector needs the minimum image
+    const double rl = p->cut_off + p->skin;
+    if (v.rbl || p->pair_path != RBMD_PAIR_LISTS) {
+      delete s;
+      return fail(RBMD_ERR_UNSUPPORTED, "a list skin is built for the VERLET lists path (pair_path lists)");
+    }
+    for (int k = 0; k < 3; ++k)
+      if (v.box[k] < 2 * rl) {
+        delete s;
+        return fail(RBMD_ERR_UNSUPPORTED, "a list skin needs box edges >= 2 (cut_off + skin) = %g", 2 * rl);
+      }
+    rbmd_sys::Geom g{};
+    g.ncell = 1;
+    for (int k = 0; k < 3; ++k) {
+      g.dims[k] = std::max(1, (int)(v.box[k] / rl));
+      g.dxdydz[k] = v.box[k] / g.dims[k];
+      g.ncell *= g.dims[k];
+    }
+    g.n_items = g.ncell + n / kChunk + 1;
+    g.lds_bins = g.ncell <= 4096 ? 1 : 0;
+    g.list_cut2 = rl * rl;
+    s->g_skin = g;
+    s->skin = true;
+    v.half_skin2 = 0.25 * p->skin * p->skin;
+  }
+  const int ncell_max = std::max(s->g_ref.ncell, s->g_skin.ncell);
+  const int items_max = std::max(s->g_ref.n_items, s->g_skin.n_items);
   // lanes per atom in the pair kernel: enough threads to fill 256 CUs for
   // small systems, fewer lanes (longer per-lane loops) for large ones
   v.lanes_lg = n <= 60000 ? 4 : (n <= 250000 ? 3 : 2);
@@ -866,13 +946,21 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   }
   // cells and lists
   A(s->dalloc(&v.cell_of, n));
-  A(s->dalloc(&v.cell_count, v.ncell));
-  A(s->dalloc(&v.cell_start, (size_t)v.ncell + 1));
-  A(s->dalloc(&v.cursor, v.ncell));
+  A(s->dalloc(&v.cell_count, ncell_max));
+  A(s->dalloc(&v.cell_start, (size_t)ncell_max + 1));
+  A(s->dalloc(&v.cursor, ncell_max));
   A(s->dalloc(&v.point_ids, n));
   A(s->dalloc(&v.rank, n));
-  A(s->dalloc(&v.items, v.n_items));
+  A(s->dalloc(&v.items, items_max));
   A(s->dalloc(&v.n_items_dev, 1));
+  if (s->skin) {
+    A(s->dalloc(&v.x0, n));
+    A(s->dalloc(&v.y0, n));
+    A(s->dalloc(&v.z0, n));
+    A(s->dalloc(&v.img0, 3 * (size_t)n));
+    A(s->dalloc(&v.rebuild_at, 1));
+  }
+  A(s->dalloc(&v.list_builds, 1));
   A(s->dalloc(&v.misfit, n));
   A(s->dalloc(&v.misfit_list, n));
   A(s->dalloc(&v.misfit_count, 1));
@@ -1032,6 +1120,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     A(s->dalloc(&s->rows, off));
   }
   A(s->dalloc(&s->nhc, 1));
+  if (s->nhc) v.cur_step = &s->nhc->step;  // a device address: the skin kernels read the step counter there
   A(s->dalloc(&s->shake_iter, 1));
   // k space
   if (p->coulomb_type != RBMD_COULOMB_NONE && !v.lj_only) {
@@ -1069,10 +1158,12 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     return rc;
   }
   hipStream_t st = ctx->stream;
-  launch_fill_int(v.cell_count, 0, v.ncell, st);
-  launch_fill_int(v.cursor, 0, v.ncell, st);  // the one-launch scan + scatter counts from zero
+  launch_fill_int(v.cell_count, 0, ncell_max, st);
+  launch_fill_int(v.cursor, 0, ncell_max, st);  // the one-launch scan + scatter counts from zero
   launch_fill_int(v.status, 0, 1, st);
   launch_fill_int(v.img, 0, 3 * n, st);
+  launch_fill_int(v.list_builds, 0, 1, st);
+  if (s->skin) launch_fill_int(reinterpret_cast<int*>(v.rebuild_at), 0, 2, st);  // int64 zero
   launch_fill_double(s->rows, 0.0, (int)s->rows_len, st);
   launch_fill_double(v.fkx, 0.0, n, st);
   launch_fill_double(v.fky, 0.0, n, st);
@@ -1173,6 +1264,7 @@ rbmd_status rbmd_sys_upload_state(rbmd_sys* s, const double* x, const double* v,
     HIP_TRY(hipStreamSynchronize(st));
   }
   s->locator_built = s->nbr_built = false;
+  s->skin_ready = false;  // a skin list is rebuilt on the next step
   return RBMD_OK;
 }
 
@@ -1205,9 +1297,9 @@ rbmd_status rbmd_sys_download_state(rbmd_sys* s, double* x, double* v, double* f
 
 rbmd_status rbmd_locator_geometry(const rbmd_sys* s, int32_t dims[3], int32_t* nc, double dx[3]) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
-  for (int d = 0; d < 3; ++d) {
-    if (dims) dims[d] = s->v.dims[d];
-    if (dx) dx[d] = s->v.dxdydz[d];
+  for (int d = 0; d < 3; ++d) {  // the reference's locator (a skin list's cells are the MD step's own)
+    if (dims) dims[d] = s->g_ref.dims[d];
+    if (dx) dx[d] = s->g_ref.dxdydz[d];
   }
   if (nc) *nc = s->v.nc;
   return RBMD_OK;
@@ -1215,6 +1307,7 @@ rbmd_status rbmd_locator_geometry(const rbmd_sys* s, int32_t dims[3], int32_t* n
 
 rbmd_status rbmd_locator_build(rbmd_sys* s) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  RC_TRY(use_geom(s, false));  // the reference's cells; a skin list is rebuilt on the next step
   build_cells(s, true);
   s->locator_built = true;
   s->nbr_built = false;
@@ -1242,7 +1335,7 @@ rbmd_status rbmd_locator_export(rbmd_sys* s, int32_t* cell_of, int32_t* point_id
 
 rbmd_status rbmd_neighbors_build(rbmd_sys* s, int32_t cap) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
-  if (!s->locator_built) RC_TRY(rbmd_locator_build(s));
+  if (!s->locator_built || s->skin_active) RC_TRY(rbmd_locator_build(s));
   RC_TRY(ensure_nbr(s, cap));
   s->ctx->run("nbr_build", [&] { launch_nbr_build(s->v, s->ctx->stream); });
   s->nbr_built = true;
@@ -1323,6 +1416,7 @@ rbmd_status rbmd_neighbors_count(rbmd_sys* s, int64_t* entries) {
 
 rbmd_status rbmd_force_real(rbmd_sys* s, double* force, double* vir, double* elj, double* ecoul) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  RC_TRY(use_geom(s, false));  // the reference's per-step list
   if (s->fused) {
     if (!s->locator_built) RC_TRY(rbmd_locator_build(s));
   } else if (!s->nbr_built) {
@@ -1555,8 +1649,13 @@ rbmd_status rbmd_md_init(rbmd_sys* s) {
   NhcState zero;
   std::memset(&zero, 0, sizeof(zero));
   HIP_TRY(hipMemcpyAsync(s->nhc, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+  if (s->skin) {  // build the skin list now (step counter 0)
+    RC_TRY(use_geom(s, true));
+    launch_skin_request(s->v, 0, st);
+  }
   build_cells(s, true);
   gather_and_list(s);
+  s->skin_ready = s->skin;
   const double* kv = nullptr;
   RC_TRY(kvec_for_step(s, 0, &kv));
   compute_force(s, kv, 1);
@@ -1598,12 +1697,17 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool the
   if (from_ring)  // the nhc phase above advanced the device step counter
     c->run("kvec_slot", [&] { launch_kvec_slot(s->kgraph, s->kring, s->ring_R, 3 * s->P, &s->nhc->step, st); });
   const double* scale = (tc == RBMD_TEMP_NOSE_HOOVER) ? &s->nhc->factor : nullptr;
-  c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, 1); });  // also bins
+  // also bins (per-step lists: the reference's locator bins before SHAKE)
+  c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, s->skin_active ? 0 : 1); });
   if (shake) {
     c->run("shake", [&] { launch_shake(s->v, s->iv, st, s->seg_rows(SEG_SHAKE), s->shake_iter); });
     s->seg_used[SEG_SHAKE] = (s->nmol + kBlock - 1) / kBlock;
     if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 1);
   }
+  // skin list: the displacement test on the final positions, which also bins
+  // them for a rebuild; the cell and list kernels below run in full only on
+  // the steps it (or a host request) marks
+  if (s->skin_active) c->run("skin_check", [&] { launch_skin_check(s->v, st); });
   build_cells(s, false);  // the cell sort also writes the sorted copies: after SHAKE
   // the positions are final for this step: the k-space and bonded forces
   // branch off to the side stream while the list is built and the pair
@@ -1688,6 +1792,7 @@ static rbmd_status capture_step(rbmd_sys* s) {
 rbmd_status rbmd_md_step(rbmd_sys* s, int64_t n_steps) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
   if (!s->initialized) return fail(RBMD_ERR_STATE, "rbmd_md_init has not run");
+  RC_TRY(prepare_skin(s));
   for (int64_t k = 0; k < n_steps; ++k) {
     RC_TRY(md_one_step(s, k == n_steps - 1));
     if (s->p.energy_every > 0 && has_k(s) && s->step % s->p.energy_every == 0) {
@@ -1706,6 +1811,7 @@ rbmd_status rbmd_md_step(rbmd_sys* s, int64_t n_steps) {
 static rbmd_status enqueue_graph_steps(rbmd_sys* s, int64_t n_steps) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
   if (!s->initialized) return fail(RBMD_ERR_STATE, "rbmd_md_init has not run");
+  RC_TRY(prepare_skin(s));
   if (!s->graph_ok) RC_TRY(capture_step(s));
   hipStream_t st = s->ctx->stream;
   const bool ring = kring_on(s);
@@ -1757,14 +1863,16 @@ rbmd_status rbmd_md_query(rbmd_sys* s, int32_t* idle) {
 rbmd_status rbmd_md_thermo(rbmd_sys* s, rbmd_thermo* o) {
   if (!s || !o) return fail(RBMD_ERR_ARG, "bad arguments");
   NhcState S;
-  int stt = 0, it = 0, mf = 0;
+  int stt = 0, it = 0, mf = 0, lb = 0;
   HIP_TRY(hipMemcpyAsync(&mf, s->v.misfit_count, 4, hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipMemcpyAsync(&S, s->nhc, sizeof(S), hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipMemcpyAsync(&stt, s->v.status, 4, hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipMemcpyAsync(&it, s->shake_iter, 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipMemcpyAsync(&lb, s->v.list_builds, 4, hipMemcpyDeviceToHost, s->ctx->stream));
   HIP_TRY(hipStreamSynchronize(s->ctx->stream));
   fill_thermo(s, S, stt, it, o);
   o->misfit_atoms = mf;
+  o->list_builds = lb;
   return RBMD_OK;
 }
 
@@ -1917,6 +2025,7 @@ rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n,
     if (s->ctx != ctx) return fail(RBMD_ERR_ARG, "system %d belongs to another context", k);
     if (!s->initialized) return fail(RBMD_ERR_STATE, "system %d: rbmd_md_init has not run", k);
     if (s->v.rbl || s->fused) return fail(RBMD_ERR_UNSUPPORTED, "system %d: packs run the VERLET list path", k);
+    if (s->skin) return fail(RBMD_ERR_UNSUPPORTED, "system %d: packs build their lists every step (skin 0)", k);
     if (s->ntypes > kMaxTypes) return fail(RBMD_ERR_UNSUPPORTED, "system %d: more than %d types", k, kMaxTypes);
     const int tc = s->p.temp_ctrl;
     if (tc != RBMD_TEMP_NOSE_HOOVER && tc != RBMD_TEMP_NONE)

@@ -109,7 +109,27 @@ struct SysView {
   double* vir_k;     // [6n] or null
   // status word: bit0 nonfinite, bit1 shake, bit2 capacity
   int* status;
+  // Verlet list with a skin (skin_on): the list holds every pair within
+  // rc + skin (list_cut2 = (rc + skin)^2; without a skin list_cut2 = cut2) and
+  // is rebuilt when an atom has moved more than skin / 2 since the build.
+  // Between builds atoms keep their sorted order and spos holds their
+  // positions unwrapped to the build's images: x + (img - img0) L.  The
+  // conditional kernels of a step do their work when *rebuild_at equals the
+  // device step counter *cur_step.
+  double list_cut2;
+  int skin_on;
+  double half_skin2;        // (skin / 2)^2
+  const int64_t* cur_step;  // the NHC state's step counter
+  int64_t* rebuild_at;      // step of the last requested rebuild
+  double *x0, *y0, *z0;     // positions at the last build (original order)
+  int* img0;                // image flags at the last build [3n]
+  int* list_builds;         // cell sorts (list builds) since creation
 };
+
+// the list and cells of this step are (re)built: always without a skin
+__device__ __forceinline__ bool list_rebuild(const SysView& s) {
+  return !s.skin_on || *s.rebuild_at == *s.cur_step;
+}
 
 // Thermo scalars reduced from per-workgroup partial rows.  Each producer
 // writes rows of its own width holding consecutive slots:
@@ -150,6 +170,11 @@ void launch_cell_sort(const SysView& s, hipStream_t st);
 bool scan_in_scatter(const SysView& s);  // few cells: the scan runs inside the scatter launch
 void launch_scan_scatter(const SysView& s, hipStream_t st);
 void launch_nbr_build(const SysView& s, hipStream_t st);
+// skin lists: flag a rebuild when an atom has moved more than skin / 2 since the
+// last build, and bin every atom (the counts feed the scan of a rebuild step)
+void launch_skin_check(const SysView& s, hipStream_t st);
+// request a rebuild at the device step counter's value + ahead
+void launch_skin_request(const SysView& s, int ahead, hipStream_t st);
 void launch_fill_int(int* p, int v, int n, hipStream_t st);
 void launch_fill_double(double* p, double v, int n, hipStream_t st);
 

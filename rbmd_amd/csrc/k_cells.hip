@@ -60,6 +60,52 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysView s, const double* __restr
 
 size_t bin_lds_bytes(const SysView& s) { return s.lds_bins ? (size_t)s.ncell * sizeof(int) : 0; }
 
+// Skin lists: after the step's positions are final (after SHAKE), flag a
+// rebuild when any atom is more than skin / 2 from its position at the last
+// build (positions unwrapped with the image flags; a non-finite distance
+// counts as moved), and bin every atom with the current positions, as the
+// rebuild's scan needs.  On a step without a rebuild the scan kernel clears
+// the counts again.  One atomic per wavefront that saw a moved atom.
+__global__ __launch_bounds__(kBlock) void k_skin_check(SysView s) {
+  extern __shared__ int lds_cells[];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int cell = -1;
+  bool moved = false;
+  if (i < s.n) {
+    const double px = s.x[i], py = s.y[i], pz = s.z[i];
+    if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) atomicOr(s.status, 1);
+    const double ux = (px + (double)(s.img[3 * i] - s.img0[3 * i]) * s.box[0]) - s.x0[i];
+    const double uy = (py + (double)(s.img[3 * i + 1] - s.img0[3 * i + 1]) * s.box[1]) - s.y0[i];
+    const double uz = (pz + (double)(s.img[3 * i + 2] - s.img0[3 * i + 2]) * s.box[2]) - s.z0[i];
+    moved = !(ux * ux + uy * uy + uz * uz <= s.half_skin2);
+    const int a = bin_axis(px, s.lo[0], s.dxdydz[0], s.dims[0]);
+    const int b = bin_axis(py, s.lo[1], s.dxdydz[1], s.dims[1]);
+    const int c = bin_axis(pz, s.lo[2], s.dxdydz[2], s.dims[2]);
+    cell = a + b * s.dims[0] + c * s.dims[0] * s.dims[1];
+    s.cell_of[i] = cell;
+  }
+  if (__ballot(moved) != 0ull && (threadIdx.x & (kWave - 1)) == 0)
+    atomicMax((unsigned long long*)s.rebuild_at, (unsigned long long)*s.cur_step);
+  count_cells(s, cell, lds_cells);
+}
+
+void launch_skin_check(const SysView& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_skin_check, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), bin_lds_bytes(s), st, s);
+}
+
+__global__ void k_skin_request(int64_t* rebuild_at, const int64_t* cur_step, int ahead) {
+  if (threadIdx.x == 0) *rebuild_at = *cur_step + ahead;
+}
+
+void launch_skin_request(const SysView& s, int ahead, hipStream_t st) {
+  hipLaunchKernelGGL(k_skin_request, dim3(1), dim3(kWave), 0, st, s.rebuild_at, s.cur_step, ahead);
+}
+
+// a step without a rebuild: the counts binned this step are dropped
+__device__ __forceinline__ void clear_counts(const SysView& s) {
+  for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) s.cell_count[c] = 0;
+}
+
 void launch_bin(const SysView& s, hipStream_t st, const double* x, const double* y, const double* z) {
   hipLaunchKernelGGL(k_bin, dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), bin_lds_bytes(s), st, s, x, y, z);
 }
@@ -137,6 +183,10 @@ __device__ __forceinline__ void cell_scan(const SysView& s, int* start, const bo
 }
 
 __device__ __forceinline__ void scan_body(const SysView& s, const int bid) {
+  if (!list_rebuild(s)) {
+    clear_counts(s);
+    return;
+  }
   cell_scan<true>(s, nullptr, true);
 }
 
@@ -153,6 +203,7 @@ void launch_scan(const SysView& s, hipStream_t st) {
 // by their LDS rank; the order inside a cell is fixed afterwards by k_cell_sort.
 __device__ __forceinline__ void scatter_body(const SysView& s, const int bid) {
   extern __shared__ int lds[];
+  if (!list_rebuild(s)) return;
   int i = bid * blockDim.x + threadIdx.x;
   int cell = i < s.n ? s.cell_of[i] : -1;
   if (!s.lds_bins) {
@@ -193,6 +244,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_scatter(SysView s) {
   int* cnt = lds + s.ncell;
   int* base = cnt + s.ncell;
   const int bid = (int)blockIdx.x;
+  if (!list_rebuild(s)) {
+    if (bid == 0) clear_counts(s);
+    return;
+  }
   cell_scan<false>(s, start, bid == 0);
   const int i = bid * blockDim.x + threadIdx.x;
   const int cell = i < s.n ? s.cell_of[i] : -1;
@@ -225,12 +280,25 @@ constexpr int kSortCap = 4096;
 // positions are final.
 __device__ __forceinline__ void cell_sort_body(const SysView& s, const int bid) {
   __shared__ int ids[kSortCap];
+  if (!list_rebuild(s)) {
+    // skin list between builds: the build's sorted order is kept; write the
+    // sorted copies of the positions, unwrapped to the build's images
+    const int stride = s.ncell * blockDim.x;
+    for (int p = bid * blockDim.x + threadIdx.x; p < s.n; p += stride) {
+      const int me = s.point_ids[p];
+      s.spos[p] = make_double4(s.x[me] + (double)(s.img[3 * me] - s.img0[3 * me]) * s.box[0],
+                               s.y[me] + (double)(s.img[3 * me + 1] - s.img0[3 * me + 1]) * s.box[1],
+                               s.z[me] + (double)(s.img[3 * me + 2] - s.img0[3 * me + 2]) * s.box[2], s.q[me]);
+    }
+    return;
+  }
   int c = bid;
   int start = s.cell_start[c];
   int cnt = s.cell_start[c + 1] - start;
   if (threadIdx.x == 0) {  // read by the binning and scatter kernels before this one
     s.cell_count[c] = 0;
     s.cursor[c] = 0;
+    if (c == 0 && s.list_builds) atomicAdd(s.list_builds, 1);
   }
   if (cnt > kSortCap) {
     if (threadIdx.x == 0) atomicOr(s.status, 4);
@@ -251,6 +319,14 @@ __device__ __forceinline__ void cell_sort_body(const SysView& s, const int bid) 
     const double px = s.x[me], py = s.y[me], pz = s.z[me];
     s.spos[p] = make_double4(px, py, pz, s.q[me]);
     s.stype[p] = s.type[me];
+    if (s.skin_on) {  // the build's positions and images, for the skin check
+      s.x0[me] = px;
+      s.y0[me] = py;
+      s.z0[me] = pz;
+      s.img0[3 * me] = s.img[3 * me];
+      s.img0[3 * me + 1] = s.img[3 * me + 1];
+      s.img0[3 * me + 2] = s.img[3 * me + 2];
+    }
     const int ua = cell_axis(px, s.lo[0], s.dxdydz[0]);
     const int ub = cell_axis(py, s.lo[1], s.dxdydz[1]);
     const int uc = cell_axis(pz, s.lo[2], s.dxdydz[2]);
@@ -318,7 +394,7 @@ __device__ void misfit_block(const SysView& s, int wave, int lane, int slot) {
         COLUMN_CANDIDATE(t, pj, entry);
         double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
         double d2 = dist2(rx, ry, rz);
-        ok = s.cut2 - d2 > 0.0001 && d2 > 0.0001;
+        ok = s.list_cut2 - d2 > 0.0001 && d2 > 0.0001;
       }
       const unsigned long long bal = __ballot(ok);
       if (ok) {
@@ -342,6 +418,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = bid / kNbrParts;
   const int wave = (bid % kNbrParts) * kNbrWaves + w;  // the stencil column this wavefront owns
+  if (!list_rebuild(s)) return;  // skin list between builds
   if (wid >= s.n_items) {
     misfit_block(s, wave, lane, wid - s.n_items);
     return;
@@ -388,7 +465,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   Column col;
   make_column(s, ca, cb, cc, a, b, lane, col);
   const int total = col.total;
-  const double cut2 = s.cut2;
+  const double cut2 = s.list_cut2;  // rc^2, or (rc + skin)^2 for a skin list
   int cnt = 0;
   // software pipeline: the candidate of the next round is loaded ahead
   double4 nxt = make_double4(0, 0, 0, 0);

@@ -167,17 +167,26 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
         }
         double sc, elj, ec;
         // list entries satisfy the acceptance test with this same vector;
-        // with the minimum image applied the range is tested again
+        // with the minimum image applied the range is tested again, and a
+        // skin list's entries (out to rc + skin) meet the list test here
         if constexpr (F32) {
           float el32, ec32;
           const float qq = (float)(pi.w * p.w);
-          sc = s.use_min_image ? pair_term32<MODE, EN, true>(s, ctab, d2, qq, epsij, sig6, (float)w, el32, ec32)
-                               : pair_term32<MODE, EN, false>(s, ctab, d2, qq, epsij, sig6, (float)w, el32, ec32);
+          if (s.use_min_image)
+            sc = pair_term32<MODE, EN, 1>(s, ctab, d2, qq, epsij, sig6, (float)w, el32, ec32);
+          else if (s.skin_on)
+            sc = pair_term32<MODE, EN, 2>(s, ctab, d2, qq, epsij, sig6, (float)w, el32, ec32);
+          else
+            sc = pair_term32<MODE, EN, 0>(s, ctab, d2, qq, epsij, sig6, (float)w, el32, ec32);
           elj = el32;
           ec = ec32;
         } else {
-          sc = s.use_min_image ? pair_term<MODE, EN, true>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec)
-                               : pair_term<MODE, EN, false>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec);
+          if (s.use_min_image)
+            sc = pair_term<MODE, EN, 1>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec);
+          else if (s.skin_on)
+            sc = pair_term<MODE, EN, 2>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec);
+          else
+            sc = pair_term<MODE, EN, 0>(s, ctab, d2, pi.w * p.w, epsij, sig6, w, elj, ec);
         }
         // one fused rounding per force term
         fx = fma(sc, rx, fx);

@@ -88,19 +88,28 @@ __device__ __forceinline__ double coul_poly(const double2* ctab, int k, double u
 //   coul   f = -qq G(r)/r = -qq P(r) rinv^3,     e = qq Q(r) rinv / 2
 //   weight f += (1 - w) qq rinv^3,               e -= (1 - w) qq rinv / 2
 // P and Q come from the LDS copy of the piecewise polynomial table.  CHECK:
-// test 1e-4 < d2 < rc^2 (off for pairs already accepted by the list predicate
-// with the same vector).
+// 0 no test (pairs the list predicate accepted with the same vector); 1 the
+// force function's own test 1e-4 < d2 < rc^2 (minimum-image vectors); 2 the
+// list predicate rc^2 - d2 > 1e-4 and d2 > 1e-4 (entries of a skin list,
+// which holds pairs out to rc + skin).  A pair that fails contributes zero.
 //
 // The force functions compile with FMA contraction off and their fused
 // multiply-adds written out: the compiler would otherwise choose what to fuse
 // per kernel (by what else reads a product), and the kernels with and without
 // the energy and virial sums, alone and packed, would round the same force
 // differently.
-template <int MODE, bool EN, bool CHECK = true>
+template <int CHECK>
+__device__ __forceinline__ bool pair_in(const SysView& s, double d2) {
+  if (CHECK == 0) return true;
+  if (CHECK == 1) return d2 < s.cut2 && d2 > 0.0001;
+  return s.cut2 - d2 > 0.0001 && d2 > 0.0001;
+}
+
+template <int MODE, bool EN, int CHECK = 1>
 __device__ __forceinline__ double pair_term(const SysView& s, const double2* ctab, double d2, double qq, double epsij,
                                             double sig6, double w, double& elj, double& ec) {
 #pragma clang fp contract(off)
-  const bool in = !CHECK || (d2 < s.cut2 && d2 > 0.0001);
+  const bool in = pair_in<CHECK>(s, d2);
   double flj = 0, fc = 0;
   elj = 0;
   ec = 0;
@@ -155,11 +164,11 @@ __device__ __forceinline__ float coul_poly32(const float2* ctab, int k, float u)
 // is the hardware reciprocal square root (v_rsq_f32; a Newton step on it left
 // the force error unchanged at ~3e-7 RMS, so none is taken).  The Coulomb
 // factors come from the degree-kCoulD32 FP32 table.
-template <int MODE, bool EN, bool CHECK = true>
+template <int MODE, bool EN, int CHECK = 1>
 __device__ __forceinline__ float pair_term32(const SysView& s, const float2* ctab, double d2d, float qq, float epsij,
                                              float sig6, float w, float& elj, float& ec) {
 #pragma clang fp contract(off)
-  const bool in = !CHECK || (d2d < s.cut2 && d2d > 0.0001);
+  const bool in = pair_in<CHECK>(s, d2d);
   const float d2 = (float)d2d;
   float flj = 0.f, fc = 0.f;
   elj = 0.f;
