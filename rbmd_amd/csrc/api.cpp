@@ -691,6 +691,23 @@ rbmd_status long_energy(rbmd_sys* s, const int kmax[3], double* e_avg, double* g
 
 }  // namespace
 
+namespace rbmd {
+
+int32_t api_fail(int32_t code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int ctx_device(const rbmd_ctx* c) { return c->device; }
+hipStream_t ctx_stream(const rbmd_ctx* c) { return c->stream; }
+
+}  // namespace rbmd
+
 // =================================================================== C ABI
 
 extern "C" {
@@ -957,7 +974,6 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
     A(s->dalloc(&v.x0, n));
     A(s->dalloc(&v.y0, n));
     A(s->dalloc(&v.z0, n));
-    A(s->dalloc(&v.img0, 3 * (size_t)n));
     A(s->dalloc(&v.rebuild_at, 1));
   }
   A(s->dalloc(&v.list_builds, 1));

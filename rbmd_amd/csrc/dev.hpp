@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct rbmd_ctx;  // the C ABI's context (include/rbmd_mi355x.h), defined in api.cpp
+
 namespace rbmd {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -112,17 +114,16 @@ struct SysView {
   // Verlet list with a skin (skin_on): the list holds every pair within
   // rc + skin (list_cut2 = (rc + skin)^2; without a skin list_cut2 = cut2) and
   // is rebuilt when an atom has moved more than skin / 2 since the build.
-  // Between builds atoms keep their sorted order and spos holds their
-  // positions unwrapped to the build's images: x + (img - img0) L.  The
-  // conditional kernels of a step do their work when *rebuild_at equals the
-  // device step counter *cur_step.
+  // Between builds atoms keep their sorted order and spos holds each build
+  // position plus the minimum-image displacement since.  The conditional
+  // kernels of a step do their work when *rebuild_at equals the device step
+  // counter *cur_step.
   double list_cut2;
   int skin_on;
   double half_skin2;        // (skin / 2)^2
   const int64_t* cur_step;  // the NHC state's step counter
   int64_t* rebuild_at;      // step of the last requested rebuild
   double *x0, *y0, *z0;     // positions at the last build (original order)
-  int* img0;                // image flags at the last build [3n]
   int* list_builds;         // cell sorts (list builds) since creation
 };
 
@@ -161,6 +162,13 @@ struct RowsDesc {
   RowSeg seg[5];
   int nseg;
 };
+
+// ---------------------------------------------------------------- host glue
+// the message of rbmd_last_error() for this thread; returns code
+int32_t api_fail(int32_t code, const char* fmt, ...);
+// a context's device and stream (the context type is the C ABI's opaque rbmd_ctx)
+int ctx_device(const ::rbmd_ctx* ctx);
+hipStream_t ctx_stream(const ::rbmd_ctx* ctx);
 
 // ---------------------------------------------------------------- cells
 void launch_bin(const SysView& s, hipStream_t st, const double* x, const double* y, const double* z);

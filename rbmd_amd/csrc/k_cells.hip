@@ -60,12 +60,22 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysView s, const double* __restr
 
 size_t bin_lds_bytes(const SysView& s) { return s.lds_bins ? (size_t)s.ncell * sizeof(int) : 0; }
 
+// Displacement of a coordinate since the last build, as the minimum image:
+// image flags cannot unwrap it (SHAKE re-applies the periodic wrap to
+// positions the drift already wrapped, so a flag can count one crossing twice,
+// as the reference's does), and between builds every displacement is below
+// skin / 2 plus one step's motion, far under half a box edge.
+__device__ __forceinline__ double skin_disp(double x, double x0, double box) {
+  const double d = x - x0;
+  return d - box * rint(d / box);
+}
+
 // Skin lists: after the step's positions are final (after SHAKE), flag a
 // rebuild when any atom is more than skin / 2 from its position at the last
-// build (positions unwrapped with the image flags; a non-finite distance
-// counts as moved), and bin every atom with the current positions, as the
-// rebuild's scan needs.  On a step without a rebuild the scan kernel clears
-// the counts again.  One atomic per wavefront that saw a moved atom.
+// build (a non-finite distance counts as moved), and bin every atom with the
+// current positions, as the rebuild's scan needs.  On a step without a
+// rebuild the scan kernel clears the counts again.  One atomic per wavefront
+// that saw a moved atom.
 __global__ __launch_bounds__(kBlock) void k_skin_check(SysView s) {
   extern __shared__ int lds_cells[];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -74,9 +84,9 @@ __global__ __launch_bounds__(kBlock) void k_skin_check(SysView s) {
   if (i < s.n) {
     const double px = s.x[i], py = s.y[i], pz = s.z[i];
     if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) atomicOr(s.status, 1);
-    const double ux = (px + (double)(s.img[3 * i] - s.img0[3 * i]) * s.box[0]) - s.x0[i];
-    const double uy = (py + (double)(s.img[3 * i + 1] - s.img0[3 * i + 1]) * s.box[1]) - s.y0[i];
-    const double uz = (pz + (double)(s.img[3 * i + 2] - s.img0[3 * i + 2]) * s.box[2]) - s.z0[i];
+    const double ux = skin_disp(px, s.x0[i], s.box[0]);
+    const double uy = skin_disp(py, s.y0[i], s.box[1]);
+    const double uz = skin_disp(pz, s.z0[i], s.box[2]);
     moved = !(ux * ux + uy * uy + uz * uz <= s.half_skin2);
     const int a = bin_axis(px, s.lo[0], s.dxdydz[0], s.dims[0]);
     const int b = bin_axis(py, s.lo[1], s.dxdydz[1], s.dims[1]);
@@ -281,14 +291,15 @@ constexpr int kSortCap = 4096;
 __device__ __forceinline__ void cell_sort_body(const SysView& s, const int bid) {
   __shared__ int ids[kSortCap];
   if (!list_rebuild(s)) {
-    // skin list between builds: the build's sorted order is kept; write the
-    // sorted copies of the positions, unwrapped to the build's images
+    // skin list between builds: the build's sorted order is kept; the sorted
+    // copy of each position is its build position plus its displacement, so
+    // a list entry's image offset still applies
     const int stride = s.ncell * blockDim.x;
     for (int p = bid * blockDim.x + threadIdx.x; p < s.n; p += stride) {
       const int me = s.point_ids[p];
-      s.spos[p] = make_double4(s.x[me] + (double)(s.img[3 * me] - s.img0[3 * me]) * s.box[0],
-                               s.y[me] + (double)(s.img[3 * me + 1] - s.img0[3 * me + 1]) * s.box[1],
-                               s.z[me] + (double)(s.img[3 * me + 2] - s.img0[3 * me + 2]) * s.box[2], s.q[me]);
+      const double x0 = s.x0[me], y0 = s.y0[me], z0 = s.z0[me];
+      s.spos[p] = make_double4(x0 + skin_disp(s.x[me], x0, s.box[0]), y0 + skin_disp(s.y[me], y0, s.box[1]),
+                               z0 + skin_disp(s.z[me], z0, s.box[2]), s.q[me]);
     }
     return;
   }
@@ -319,13 +330,10 @@ __device__ __forceinline__ void cell_sort_body(const SysView& s, const int bid) 
     const double px = s.x[me], py = s.y[me], pz = s.z[me];
     s.spos[p] = make_double4(px, py, pz, s.q[me]);
     s.stype[p] = s.type[me];
-    if (s.skin_on) {  // the build's positions and images, for the skin check
+    if (s.skin_on) {  // the build's positions, for the skin check and the gathers until the next build
       s.x0[me] = px;
       s.y0[me] = py;
       s.z0[me] = pz;
-      s.img0[3 * me] = s.img[3 * me];
-      s.img0[3 * me + 1] = s.img[3 * me + 1];
-      s.img0[3 * me + 2] = s.img[3 * me + 2];
     }
     const int ua = cell_axis(px, s.lo[0], s.dxdydz[0]);
     const int ub = cell_axis(py, s.lo[1], s.dxdydz[1]);

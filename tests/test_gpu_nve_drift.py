@@ -1,0 +1,81 @@
+"""NVE energy drift over 10,000 steps against the oracle's record.
+
+The workload and the record come from tests/golden/make_nve_drift.py (the
+oracle run once on the CPU): salt, 4,000 ions, plain velocity Verlet
+(temp_ctrl NONE, the reference's NVT step with the thermostat decoupled,
+ExecutionNVT.cpp:1112-1116; its NVE is a stub, ExecutionNVE.cpp:36-41), with
+Ewald (deterministic force) and with RBE (P = 100, the same seeded k sets).
+The total energy per atom is recorded every 100 steps; the drift is the
+least-squares slope in energy per atom per time unit and the spread
+max |E - E0| / |E0|.
+
+The two runs start from the same state and the first records agree to
+1e-10; after that the trajectories separate (round-off grows exponentially
+in a chaotic system), so the drift and the spread are compared as
+statistics: the GPU's must lie within the stated bands of the oracle's.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from rbmd_amd import make_salt, salt_config
+from rbmd_amd.lib import GpuSystem
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "nve_drift_salt4000.json")
+
+
+def _load():
+    with open(FIXTURE) as f:
+        return json.load(f)
+
+
+def _series(ctx, coul, fx):
+    run = salt_config(coulomb_type=coul, kmax=fx["kmax"], temp_ctrl="NONE", energy_every=fx["every"])
+    g = GpuSystem(ctx, make_salt(n_ions=4000, seed=3), run)
+    g.md_init()
+    n = 4000
+
+    def e_of(t):
+        return 0.5 * t["ke_sum"] / n + t["e_lj"] + t["e_coul"] + t["e_long"]
+
+    steps, e = [0], [e_of(g.thermo())]
+    for _ in range(fx["steps"] // fx["every"]):
+        g.md_step(fx["every"])
+        t = g.thermo()
+        assert t["energy_step"] == t["step"] and t["status_flags"] == 0
+        steps.append(t["step"])
+        e.append(e_of(t))
+    g.close()
+    tt = np.asarray(steps, float) * run.dt
+    e = np.asarray(e)
+    return e, float(np.polyfit(tt, e, 1)[0]), float(np.abs(e - e[0]).max() / abs(e[0]))
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("coul", ["EWALD", "RBE"])
+def test_nve_drift_matches_oracle(ctx, coul):
+    fx = _load()
+    ref = fx["runs"][coul]
+    print(f"[case] NVE drift {coul}: {fx['steps']} steps", flush=True)
+    e, slope, spread = _series(ctx, coul, fx)
+    e_ref = np.asarray(ref["energy"])
+    assert abs(e[0] - e_ref[0]) <= 1e-10 * abs(e_ref[0])
+    assert abs(e[1] - e_ref[1]) <= 1e-8 * abs(e_ref[1])
+    print(f"[case] slope gpu {slope:.4e} oracle {ref['slope']:.4e}; spread gpu {spread:.3e} oracle "
+          f"{ref['spread']:.3e}", flush=True)
+    if coul == "EWALD":
+        # energy is conserved: both drifts are tiny against the energy scale,
+        # and their difference too
+        assert abs(slope) < 1e-4 * abs(e_ref[0]) and abs(ref["slope"]) < 1e-4 * abs(e_ref[0])
+        assert abs(slope - ref["slope"]) < 2e-5 * abs(e_ref[0])
+        assert spread < 3 * ref["spread"] + 1e-6
+    else:
+        # the RBE force's variance heats the system: the same heating rate
+        assert ref["slope"] > 0
+        assert abs(slope - ref["slope"]) <= 0.2 * abs(ref["slope"])
+        assert 0.5 * ref["spread"] <= spread <= 2.0 * ref["spread"]

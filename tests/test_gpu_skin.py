@@ -31,10 +31,12 @@ def test_skin_md_matches_oracle(ctx, kind):
 
 
 def test_skin_rebuilds_on_displacement(ctx):
-    """over 300 water steps the list is rebuilt a few times, not every step, and
-    the state still matches the per-step path"""
+    """over 300 water steps the list is rebuilt when atoms have moved, not every
+    step (this box relaxes from its lattice start to ~950 K, where the fastest
+    hydrogen moves ~0.25 A per step: a 2 A skin lasts a few steps), and the
+    state still matches the per-step path"""
     s = water()
-    g1 = GpuSystem(ctx, s, water_run(skin=1.0))
+    g1 = GpuSystem(ctx, s, water_run(skin=2.0))
     g2 = GpuSystem(ctx, s, water_run())
     g1.md_init()
     g2.md_init()
@@ -42,7 +44,7 @@ def test_skin_rebuilds_on_displacement(ctx):
         g1.md_step(100)
         g2.md_step(100)
     t1, t2 = g1.thermo(), g2.thermo()
-    assert 2 <= t1["list_builds"] <= 150, t1["list_builds"]
+    assert 10 <= t1["list_builds"] <= 150, t1["list_builds"]
     assert t2["list_builds"] == 301
     x1, v1, f1, _ = g1.state()
     x2, v2, f2, _ = g2.state()
