@@ -66,16 +66,21 @@ def test_nve_drift_matches_oracle(ctx, coul):
     e_ref = np.asarray(ref["energy"])
     assert abs(e[0] - e_ref[0]) <= 1e-10 * abs(e_ref[0])
     assert abs(e[1] - e_ref[1]) <= 1e-8 * abs(e_ref[1])
-    print(f"[case] slope gpu {slope:.4e} oracle {ref['slope']:.4e}; spread gpu {spread:.3e} oracle "
-          f"{ref['spread']:.3e}", flush=True)
+    # the standard error of the oracle's fitted slope; two realizations that
+    # have decorrelated differ by ~sqrt(2) of it, and the band is 5 of those
+    t = np.asarray(ref["steps"], float) * 0.001
+    fit = np.polyfit(t, e_ref, 1)
+    res = e_ref - np.polyval(fit, t)
+    stderr = np.sqrt((res ** 2).sum() / (len(t) - 2)) / np.sqrt(((t - t.mean()) ** 2).sum())
+    band = 5 * np.sqrt(2) * stderr
+    print(f"[case] slope gpu {slope:.4e} oracle {ref['slope']:.4e} (band {band:.2e}); spread gpu {spread:.3e} "
+          f"oracle {ref['spread']:.3e}", flush=True)
+    assert abs(slope - ref["slope"]) <= band
+    assert 0.5 * ref["spread"] <= spread <= 2.0 * ref["spread"]
     if coul == "EWALD":
-        # energy is conserved: both drifts are tiny against the energy scale,
-        # and their difference too
-        assert abs(slope) < 1e-4 * abs(e_ref[0]) and abs(ref["slope"]) < 1e-4 * abs(e_ref[0])
-        assert abs(slope - ref["slope"]) < 2e-5 * abs(e_ref[0])
-        assert spread < 3 * ref["spread"] + 1e-6
+        # energy is conserved: the drift is tiny against the energy itself
+        # (oracle: -2.7e-5 per atom per tau at E = -1.84)
+        assert abs(slope) < 1e-4 * abs(e_ref[0])
     else:
-        # the RBE force's variance heats the system: the same heating rate
-        assert ref["slope"] > 0
-        assert abs(slope - ref["slope"]) <= 0.2 * abs(ref["slope"])
-        assert 0.5 * ref["spread"] <= spread <= 2.0 * ref["spread"]
+        # the RBE force's variance heats the system at a clear rate (+1.9e-3)
+        assert slope > 10 * band
