@@ -39,8 +39,13 @@ def converged_kmax(alpha, box, ktol=1e-12):
     return [int(math.ceil(kc * float(L) / (2.0 * math.pi))) for L in box]
 
 
-def rbe_method_error(ctx, system, run, n_sets=16, seed=2024, ktol=1e-12, kmax=None):
-    """RBE k-space force of n_sets independent k sets against converged Ewald."""
+def rbe_method_error(ctx, system, run, n_sets=16, seed=2024, ktol=1e-12, kmax=None, checkpoints=()):
+    """RBE k-space force of n_sets independent k sets against converged Ewald.
+
+    checkpoints: set counts M (<= n_sets) at which the error of the average
+    over the first M sets is also reported ("averaged_rel_total_by_m"), with
+    the least-squares slope of log(error) against log(M) and the M at which
+    that line reaches a relative error of 1e-3."""
     g = GpuSystem(ctx, system, run)
     try:
         km = kmax or converged_kmax(run.alpha, system.box, ktol)
@@ -50,13 +55,17 @@ def rbe_method_error(ctx, system, run, n_sets=16, seed=2024, ktol=1e-12, kmax=No
         ftot = fr + fe
         acc = np.zeros_like(fe)
         per = []
+        by_m = {}
+        marks = set(int(m) for m in checkpoints if 0 < int(m) <= n_sets)
         for s in range(n_sets):
             g.set_kvectors(rbe_sample_host(run.alpha, system.box, run.rbe_p, seed, s + 1))
             fk, _ = g.rbe_force()
             per.append(rms(fk - fe))
             acc += fk
+            if s + 1 in marks:
+                by_m[s + 1] = rms(acc / (s + 1) - fe) / rms(ftot)
         avg = acc / n_sets
-        return {
+        out = {
             "atoms": system.n_atoms, "P": run.rbe_p, "alpha": run.alpha, "kmax": km, "k_sets": n_sets,
             "rms_force_total": rms(ftot), "rms_force_ewald_k": rms(fe),
             "per_step_rel_total": float(np.mean(per)) / rms(ftot),
@@ -64,6 +73,15 @@ def rbe_method_error(ctx, system, run, n_sets=16, seed=2024, ktol=1e-12, kmax=No
             "averaged_rel_total": rms(avg - fe) / rms(ftot),
             "averaged_rel_kspace": rms(avg - fe) / rms(fe),
         }
+        if by_m:
+            out["averaged_rel_total_by_m"] = {str(m): by_m[m] for m in sorted(by_m)}
+        if len(by_m) >= 2:
+            lm = np.log(np.array(sorted(by_m), dtype=float))
+            le = np.log(np.array([by_m[m] for m in sorted(by_m)]))
+            slope, icpt = np.polyfit(lm, le, 1)
+            out["log_slope"] = float(slope)
+            out["m_for_1e-3"] = float(np.exp((math.log(1e-3) - icpt) / slope)) if slope < 0 else None
+        return out
     finally:
         g.close()
 

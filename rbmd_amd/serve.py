@@ -370,3 +370,71 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
             "atom_steps_per_s": done_atom_steps / wall if wall > 0 else 0.0,
             "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None, "mode": mode}
+
+
+def serve_plain(jobs: List[JobSpec], device: int = 0, time_scale: float = 1.0, prebuilt: Optional[dict] = None,
+                results: Optional[dict] = None, progress=None) -> dict:
+    """The same job stream run the plain way: one system at a time on one
+    stream, in arrival order, FP64, eager steps (rbmd_md_step), each job set up
+    when the previous one has finished.  Arrivals follow the same clock as
+    serve(); latency = finish - arrival, so jobs that arrive while another runs
+    wait in the queue.  `prebuilt` maps a job index to its (System, RunConfig)
+    (the precision is forced to FP64)."""
+    order = sorted(range(len(jobs)), key=lambda i: jobs[i].arrival)
+    ctx = Context(device)
+    lat = []
+    busy = 0.0
+    done_atom_steps = 0
+    t_start = time.perf_counter()
+    for j in order:
+        now = time.perf_counter() - t_start
+        wait = jobs[j].arrival * time_scale - now
+        if wait > 0:
+            time.sleep(wait)
+        s, r = prebuilt[j] if prebuilt is not None else build(jobs[j], "fp64")
+        if r.precision != "fp64":
+            r = type(r)(**{**r.__dict__, "precision": "fp64"})
+        t0 = time.perf_counter()
+        g = GpuSystem(ctx, s, r)
+        g.md_init()
+        g.md_step(jobs[j].steps)
+        ctx.synchronize()
+        if results is not None:
+            results[j] = g.state()
+        g.close()
+        t1 = time.perf_counter()
+        busy += t1 - t0
+        lat.append((t1 - t_start) - jobs[j].arrival * time_scale)
+        done_atom_steps += jobs[j].atoms * jobs[j].steps
+        if progress is not None and len(lat) % 10 == 0:
+            progress(len(lat))
+    wall = time.perf_counter() - t_start
+    ctx.close()
+    lat = np.sort(np.asarray(lat))
+    return {"jobs": len(jobs), "wall_s": wall, "busy_s": busy, "atom_steps": done_atom_steps,
+            "atom_steps_per_s": done_atom_steps / wall if wall > 0 else 0.0,
+            "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None, "mode": "plain"}
+
+
+def _build_one(args):
+    spec, precision = args
+    return build(spec, precision)
+
+
+def build_parallel(specs: List[JobSpec], precision: str = "fp64", workers: int = 16, progress=None) -> list:
+    """build_all over a pool of fresh interpreters (spawned, so no GPU state is
+    inherited); the same inputs as build_all, in order."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    workers = max(1, min(workers, len(specs)))
+    if workers == 1:
+        return build_all(specs, precision, progress)
+    out = []
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        for i, r in enumerate(ex.map(_build_one, [(sp, precision) for sp in specs], chunksize=4)):
+            out.append(r)
+            if progress is not None and (i + 1) % 64 == 0:
+                progress(i + 1)
+    return out

@@ -33,24 +33,37 @@ def test_skin_md_matches_oracle(ctx, kind):
 def test_skin_rebuilds_on_displacement(ctx):
     """over 300 water steps the list is rebuilt when atoms have moved, not every
     step (this box relaxes from its lattice start to ~950 K, where the fastest
-    hydrogen moves ~0.25 A per step: a 2 A skin lasts a few steps), and the
-    state still matches the per-step path"""
+    hydrogen moves ~0.25 A per step: a 2 A skin lasts a few steps); then, on a
+    step without a rebuild, the step's own force (summed over a list kept for
+    several steps) equals the force of a list built on the same positions.
+    Ewald k space makes the force a function of the positions alone.  The two
+    paths' trajectories are not compared this far out: their summation-order
+    differences grow exponentially in a chaotic system (the 20-step comparison
+    above bounds them while they are small)."""
     s = water()
-    g1 = GpuSystem(ctx, s, water_run(skin=2.0))
-    g2 = GpuSystem(ctx, s, water_run())
+    r = water_run(skin=2.0, coulomb_type="EWALD", kmax=[6, 6, 6])
+    g1 = GpuSystem(ctx, s, r)
+    print("[case] skin water 300 steps, Ewald", flush=True)
     g1.md_init()
-    g2.md_init()
     for _ in range(3):
         g1.md_step(100)
-        g2.md_step(100)
-    t1, t2 = g1.thermo(), g2.thermo()
+    t1 = g1.thermo()
     assert 10 <= t1["list_builds"] <= 150, t1["list_builds"]
-    assert t2["list_builds"] == 301
+    assert t1["status_flags"] == 0
+    kept = False
+    for _ in range(20):  # a step that kept the list
+        before = g1.thermo()["list_builds"]
+        g1.md_step(1)
+        if g1.thermo()["list_builds"] == before:
+            kept = True
+            break
+    assert kept
     x1, v1, f1, _ = g1.state()
-    x2, v2, f2, _ = g2.state()
-    # chaotic growth of summation-order differences stays far below the state's scale
-    assert rel_err(f1, f2) < 1e-4
-    assert abs(t1["temperature"] - t2["temperature"]) < 1e-5 * t2["temperature"]
+    cur = type(s)(**{**s.__dict__, "x": x1, "v": v1})
+    g2 = GpuSystem(ctx, cur, water_run(coulomb_type="EWALD", kmax=[6, 6, 6]))
+    g2.md_init()  # the force of a list built on these positions
+    _, _, f2, _ = g2.state()
+    assert rel_err(f1, f2) < 1e-10
 
 
 @pytest.mark.parametrize("kind", ["salt", "water"])
