@@ -202,12 +202,12 @@ def _alone(ctx, s, r, steps):
 @pytest.mark.timeout(1200)
 def test_ensemble_512_full_size(ctx):
     from rbmd_amd.lib import Context
-    from rbmd_amd.serve import PackGroup, build_all, ensemble_specs
+    from rbmd_amd.serve import PackGroup, build_parallel, ensemble_specs
 
     _threads()
     specs = ensemble_specs(512, 500, seed=2024)
     print(f"[case] ensemble 512 systems, {sum(sp.atoms for sp in specs)} atoms", flush=True)
-    inputs = build_all(specs, "mixed")
+    inputs = build_parallel(specs, "mixed")
     ctxs = {k: Context(0) for k in ("salt", "water")}
     groups = {k: PackGroup(ctxs[k]) for k in ctxs}
     gs = []

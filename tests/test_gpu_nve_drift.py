@@ -82,5 +82,6 @@ def test_nve_drift_matches_oracle(ctx, coul):
         # (oracle: -2.7e-5 per atom per tau at E = -1.84)
         assert abs(slope) < 1e-4 * abs(e_ref[0])
     else:
-        # the RBE force's variance heats the system at a clear rate (+1.9e-3)
-        assert slope > 10 * band
+        # the RBE force's variance heats the system at a clear rate (oracle
+        # +1.9e-3, ~28 standard errors of its fit)
+        assert slope > 10 * stderr
