@@ -267,6 +267,16 @@ rbmd_status rbmd_md_wait(rbmd_sys* sys);
 rbmd_status rbmd_md_query(rbmd_sys* sys, int32_t* idle);
 rbmd_status rbmd_md_thermo(rbmd_sys* sys, rbmd_thermo* out);
 rbmd_status rbmd_md_energy(rbmd_sys* sys); /* evaluate e_long now (ComputeEwaldEnergy) */
+/* rbmd_md_energy without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and result, bit for bit; the k grid and the structure-factor
+ * partials go to a fixed per-device scratch of 32 MiB held until the process
+ * ends, so a kmax grid that needs more (8 bytes x (3 K + 2 K + 2 K x atom
+ * blocks), K = (2 kmax + 1)^3 - 1) returns RBMD_ERR_ARG.  Calls on one device
+ * share the scratch: calls that overlap on one device (two contexts, two
+ * threads) race. */
+rbmd_status rbmd_md_energy_noalloc(rbmd_sys* sys);
 
 /* ---- packs: many systems, one launch per kernel of the step.  No counterpart
  * in the reference (its executioner, Executioner.cpp:62-129, runs one system);

@@ -18,7 +18,7 @@ LIB = os.path.join(OUT, "librbmd_mi355x.so")
 ARCH = os.environ.get("RBMD_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["k_cells.hip", "k_force.hip", "k_pair.hip", "k_kspace.hip", "k_integrate.hip", "api.cpp", "sampler.cpp"]
+SOURCES = ["k_cells.hip", "k_force.hip", "k_pair.hip", "k_kspace.hip", "k_integrate.hip", "noalloc.hip", "api.cpp", "sampler.cpp"]
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 

@@ -640,8 +640,13 @@ void fill_thermo(const rbmd_sys* s, const NhcState& S, int status, int max_iter,
 }
 
 // e_long = qqr2e (2 pi / V) sum_grid e^{-k^2/4a}|rho|^2/k^2 / N + self / N
-// (ComputeEwaldEnergy + ComputeSelfEnergy, ExecutionMD.cpp:405-443, 508-558)
-rbmd_status long_energy(rbmd_sys* s, const int kmax[3], double* e_avg, double* grid_sum_out) {
+// (ComputeEwaldEnergy + ComputeSelfEnergy, ExecutionMD.cpp:405-443, 508-558).
+// The k grid, the structure-factor partials and the reduced rho live in
+// device memory allocated for the call, or, with `scratch` (the no-alloc
+// entry point), in that buffer of scratch_bytes, which every launch below
+// overwrites in full before reading it (no clearing needed).
+rbmd_status long_energy(rbmd_sys* s, const int kmax[3], double* e_avg, double* grid_sum_out,
+                        void* scratch = nullptr, size_t scratch_bytes = 0) {
   std::vector<double> kg;
   for (int i = -kmax[0]; i <= kmax[0]; ++i)
     for (int j = -kmax[1]; j <= kmax[1]; ++j)
@@ -652,25 +657,41 @@ rbmd_status long_energy(rbmd_sys* s, const int kmax[3], double* e_avg, double* g
           kg.push_back(2 * kPi * k / s->v.box[2]);
         }
   int K = (int)(kg.size() / 3);
-  double* kdev = nullptr;
-  HIP_TRY(hipMalloc(&kdev, kg.size() * sizeof(double)));
-  double *rp = nullptr, *rr = nullptr;
   const int atoms = rho_atoms_per_block(s->n, rho_kblocks(s->v, K));
   const int nblk = (s->n + atoms - 1) / atoms;
-  HIP_TRY(hipMalloc(&rp, (size_t)nblk * 2 * K * sizeof(double)));
-  HIP_TRY(hipMalloc(&rr, (size_t)2 * K * sizeof(double)));
-  HIP_TRY(hipMemcpyAsync(kdev, kg.data(), kg.size() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
-  KView k{K, 1.0, kdev, rp, rr, nblk, 0, nullptr, 1, atoms};  // energy only: no force records
-  launch_rho_partial(s->v, k, s->ctx->stream);
-  launch_rho_reduce(s->v, k, s->ctx->stream);
+  // three slices, each starting on a 256-byte boundary
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t b_k = up(kg.size() * sizeof(double)), b_rp = up((size_t)nblk * 2 * K * sizeof(double)),
+               b_rr = up((size_t)2 * K * sizeof(double));
+  double *kdev = nullptr, *rp = nullptr, *rr = nullptr;
+  if (scratch) {
+    if (b_k + b_rp + b_rr > scratch_bytes)
+      return fail(RBMD_ERR_ARG,
+                  "the kmax grid (%d k vectors over %d atom blocks) needs %zu bytes of scratch; the no-alloc "
+                  "scratch holds %zu",
+                  K, nblk, b_k + b_rp + b_rr, scratch_bytes);
+    kdev = static_cast<double*>(scratch);
+    rp = reinterpret_cast<double*>(static_cast<char*>(scratch) + b_k);
+    rr = reinterpret_cast<double*>(static_cast<char*>(scratch) + b_k + b_rp);
+  } else {
+    HIP_TRY(hipMalloc(&kdev, b_k + b_rp + b_rr));
+    rp = reinterpret_cast<double*>(reinterpret_cast<char*>(kdev) + b_k);
+    rr = reinterpret_cast<double*>(reinterpret_cast<char*>(kdev) + b_k + b_rp);
+  }
   std::vector<double> rho((size_t)2 * K);
-  HIP_TRY(hipMemcpyAsync(rho.data(), rr, rho.size() * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
   std::vector<double> q(s->n);
-  HIP_TRY(hipMemcpyAsync(q.data(), s->v.q, s->n * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
-  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
-  (void)hipFree(kdev);
-  (void)hipFree(rp);
-  (void)hipFree(rr);
+  hipError_t e = hipMemcpyAsync(kdev, kg.data(), kg.size() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream);
+  if (e == hipSuccess) {
+    KView k{K, 1.0, kdev, rp, rr, nblk, 0, nullptr, 1, atoms};  // energy only: no force records
+    launch_rho_partial(s->v, k, s->ctx->stream);
+    launch_rho_reduce(s->v, k, s->ctx->stream);
+    e = hipMemcpyAsync(rho.data(), rr, rho.size() * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(q.data(), s->v.q, s->n * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  if (!scratch) (void)hipFree(kdev);
+  if (e != hipSuccess) return fail(RBMD_ERR_HIP, "Ewald energy: %s", hipGetErrorString(e));
   double sum = 0, cmp = 0;
   for (int l = 0; l < K; ++l) {
     double kx = kg[3 * l], ky = kg[3 * l + 1], kz = kg[3 * l + 2];
@@ -721,6 +742,9 @@ rbmd_status rbmd_ctx_create(int32_t device, rbmd_ctx** out) {
   if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0) return fail(RBMD_ERR_NODEVICE, "no HIP device visible");
   if (device < 0 || device >= cnt) return fail(RBMD_ERR_ARG, "device %d out of range (%d devices)", device, cnt);
   HIP_TRY(hipSetDevice(device));
+  // the no-alloc scratch's address on this device (taken once per device)
+  const hipError_t se = scratch_init(device);
+  if (se != hipSuccess) return fail(RBMD_ERR_HIP, "scratch address on device %d: %s", device, hipGetErrorString(se));
   rbmd_ctx* c = new rbmd_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1896,6 +1920,16 @@ rbmd_status rbmd_md_energy(rbmd_sys* s) {
   if (!s) return fail(RBMD_ERR_ARG, "null sys");
   if (!has_k(s)) return fail(RBMD_ERR_STATE, "no coulomb k space");
   RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
+  s->energy_step = s->step;
+  return RBMD_OK;
+}
+
+rbmd_status rbmd_md_energy_noalloc(rbmd_sys* s) {
+  if (!s) return fail(RBMD_ERR_ARG, "null sys");
+  if (!has_k(s)) return fail(RBMD_ERR_STATE, "no coulomb k space");
+  void* scratch = scratch_address(s->ctx->device);
+  if (!scratch) return fail(RBMD_ERR_STATE, "no scratch address on device %d", s->ctx->device);
+  RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr, scratch, kScratchBytes));
   s->energy_step = s->step;
   return RBMD_OK;
 }

@@ -170,6 +170,14 @@ int32_t api_fail(int32_t code, const char* fmt, ...);
 int ctx_device(const ::rbmd_ctx* ctx);
 hipStream_t ctx_stream(const ::rbmd_ctx* ctx);
 
+// ---------------------------------------------------------------- no-alloc scratch
+// bytes of the fixed per-device scratch (noalloc.hip)
+constexpr size_t kScratchBytes = size_t(32) << 20;
+// take the scratch's device address on `device` (current); once per device
+hipError_t scratch_init(int device);
+// the scratch's device address on `device` (null before scratch_init)
+void* scratch_address(int device);
+
 // ---------------------------------------------------------------- cells
 void launch_bin(const SysView& s, hipStream_t st, const double* x, const double* y, const double* z);
 void launch_scan(const SysView& s, hipStream_t st);

@@ -24,7 +24,8 @@ EXPORTS = [
     "rbmd_locator_export", "rbmd_neighbors_build", "rbmd_neighbors_count", "rbmd_neighbors_capacity", "rbmd_neighbors_export", "rbmd_force_real",
     "rbmd_rbe_set_kvectors", "rbmd_rbe_sample", "rbmd_rbe_sample_host", "rbmd_rbe_S",
     "rbmd_rbe_structure_factor", "rbmd_rbe_force", "rbmd_ewald_force", "rbmd_force_bonded", "rbmd_md_init",
-    "rbmd_md_step", "rbmd_md_run_graph", "rbmd_md_enqueue", "rbmd_md_wait", "rbmd_md_query", "rbmd_md_thermo", "rbmd_md_energy", "rbmd_profile_enable",
+    "rbmd_md_step", "rbmd_md_run_graph", "rbmd_md_enqueue", "rbmd_md_wait", "rbmd_md_query", "rbmd_md_thermo", "rbmd_md_energy",
+    "rbmd_md_energy_noalloc", "rbmd_profile_enable",
     "rbmd_profile_reset", "rbmd_profile_get", "rbmd_abi_sizeof", "rbmd_pack_create", "rbmd_pack_destroy",
     "rbmd_pack_step", "rbmd_pack_enqueue", "rbmd_pack_wait", "rbmd_static_table_gnear",
 ]
@@ -81,6 +82,7 @@ def load():
     L.rbmd_md_query.argtypes = [vp, C.POINTER(C.c_int32)]
     L.rbmd_md_thermo.argtypes = [vp, C.POINTER(Thermo)]
     L.rbmd_md_energy.argtypes = [vp]
+    L.rbmd_md_energy_noalloc.argtypes = [vp]
     L.rbmd_profile_enable.argtypes = [vp, C.c_int32]
     L.rbmd_profile_reset.argtypes = [vp]
     L.rbmd_profile_get.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), pd]
@@ -341,6 +343,9 @@ class GpuSystem:
 
     def md_energy(self):
         return check(self.L.rbmd_md_energy(self.h))
+
+    def md_energy_noalloc(self):
+        return check(self.L.rbmd_md_energy_noalloc(self.h))
 
 
 class Pack:
