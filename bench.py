@@ -7,21 +7,36 @@ FP64 on the GPU.  A "step" is one full MD step (NHC half steps, kick, drift,
 cell list, SHAKE, neighbour lists, real-space LJ + erfc, RBE structure factor
 and force, bonded terms, kick, RATTLE) with all state resident in HBM.
 
-Prints one JSON line.  Extra objects:
-  roofline      dominant kernel: algorithmic FP64 flops per launch / its mean
-                duration measured with HIP events on the library's stream,
-                against the FP64 vector peak (78.6 TFLOP/s).  Algorithmic work
-                is the reference algorithm's: 8 flops per distance test of an
-                atom against every atom of its 27 stencil cells, 43 per
-                accepted pair (pair term, counted from each side)
-  cpu_baseline  the CPU oracle (a C++ restatement of the reference
-                algorithms) timed on this host's cores on a bounded sample of
-                the same workload (--cpu-steps MD steps, about 10 s)
+Prints one JSON line; `value` is the water line's atom-steps/s.  Extra objects:
+  roofline          the dominant kernel (most time per step) by algorithmic
+                    work per launch over its mean duration (HIP events on the
+                    library's streams): 43 flop per accepted pair for the pair
+                    kernel (rejected candidates and skin entries not counted),
+                    39 / 35 flop per (atom, k) term for the structure factor /
+                    k-space force, bytes for the list build and binning
+                    (kernel_work); peaks are MI355X's published FP64 vector
+                    78.6 TFLOP/s (FP32 157.3 for mixed) and HBM 8 TB/s
+  real_space_stage  list build + pair kernel per step against the 43 flop per
+                    accepted pair convention, with the reference search's
+                    candidate-test count beside it
+  force_error       RBE k-space force against converged FP64 Ewald on the
+                    run's final configuration, per step and averaged over the
+                    first M k sets (M = 16, 64, 256, 1024), with the log-log
+                    slope and the M that reaches 1e-3
+  ensemble          512 salt/water systems x 500 steps: packs (mixed) against
+                    the plain one-system-at-a-time FP64 loop (child process)
+  served            Poisson job stream of the served workload at a scaled rate:
+                    p50/p99 latency and atom-steps/s of the served loop and of
+                    the plain loop on the same arrivals (child process)
+  cpu_baseline      the CPU oracle (a C++ restatement of the reference
+                    algorithms) on this host: OpenMP at up to 16 threads and
+                    serial, on bounded samples of the same workload, with nproc
+                    and the CPU model
 
 roofline.traffic is the dominant kernel's HBM bytes per launch measured with
 rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (profiles/hbm_traffic.json, scaled by
 the same counters' reading for a device copy of known size, see
---copy-calibrate); null when that kernel was not measured.
+--copy-calibrate); null when that kernel was not measured on this workload.
 """
 import argparse
 import json
@@ -132,7 +147,8 @@ def ensemble_run(systems, steps, precision, mode="pack", streams=8, chunk=50, ma
     log(f"plain loop {plain['atom_steps_per_s']:.4g} atom-steps/s")
     if precision != "fp64":
         inputs = [(s, type(r)(**{**r.__dict__, "precision": precision})) for s, r in inputs]
-    packed = run_packed(specs, streams=streams, chunk=chunk, precision=precision, mode=mode, inputs=inputs)
+    packed = run_packed(specs, streams=streams, chunk=chunk, precision=precision, mode=mode, inputs=inputs,
+                        progress=lambda m: log(f"{mode}: {m}"))
     log(f"{mode} {packed['atom_steps_per_s']:.4g} atom-steps/s")
     return {"workload": f"{len(specs)} independent systems, half charged-LJ salt, half rigid water, atoms "
                         f"log-uniform in [600, {max_atoms}] (seed 2024), {steps} steps each",
@@ -187,19 +203,19 @@ def bench_ensemble(args):
                       "precision": args.precision, "plain_loop_precision": "fp64",
                       "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
            "plain_loop_atom_steps_per_s": e["plain_loop_atom_steps_per_s"],
-           "speedup_vs_plain_loop": e["speedup_vs_plain_loop"]}
+           "speedup_vs_plain_loop": e["speedup_vs_plain_loop"], "leg": e}
     print(json.dumps(out))
 
 
 def bench_serve(args):
     r = served_run(args.rate, args.duration, args.precision, max_atoms=args.max_atoms, max_steps=args.max_steps,
-                   mode=args.ens_mode, streams=args.streams, chunk=args.chunk)
+                   mode=args.ens_mode, streams=args.streams, chunk=args.chunk, plain=not args.no_plain_serve)
     out = {"metric": "served jobs: p50/p99 latency, aggregate atom-steps/s", "value": r["atom_steps_per_s"],
            "unit": "atom-steps/s", "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype_of(args.precision),
            "data": "synthetic", "config": {"workload": r["workload"], "mode": args.ens_mode, "chunk": args.chunk,
                                             "precision": args.precision},
-           **{k: v for k, v in r.items() if k not in ("workload", "precision", "mode", "chunk")}}
+           **{k: v for k, v in r.items() if k not in ("workload", "precision", "mode", "chunk")}, "leg": r}
     print(json.dumps(out))
 
 
@@ -266,6 +282,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="water / salt: skip the ensemble object")
     ap.add_argument("--no-serve", action="store_true", help="water / salt: skip the served-stream object")
     ap.add_argument("--no-plain-serve", action="store_true", help="serve: skip the plain-loop comparison")
+    ap.add_argument("--leg-timeout", type=int, default=420, help="seconds allowed to the ensemble and served legs")
     ap.add_argument("--cpu-steps", type=int, default=None, help="OpenMP CPU baseline steps (default 100; 10 at 300k)")
     ap.add_argument("--serial-steps", type=int, default=None, help="serial CPU baseline steps (default 8; 2 at 300k)")
     ap.add_argument("--rbe-p", type=int, default=None, help="RBE k vectors per step (default: the workload's)")
@@ -465,14 +482,16 @@ def main():
                             "extrapolates the log-log fit to an averaged error of 1e-3")
         out["force_error"] = fe
     g.close()
+    ctx.close()
     extra_ok = args.workload in ("water", "salt")
     if extra_ok and not args.no_ensemble:
-        out["ensemble"] = ensemble_run(args.systems, args.ens_steps, "mixed", mode=args.ens_mode,
-                                       streams=args.streams, chunk=args.chunk, max_atoms=args.max_atoms or 30000)
+        out["ensemble"] = child_leg(["--workload", "ensemble", "--systems", str(args.systems), "--ens-steps",
+                                     str(args.ens_steps), "--chunk", str(args.chunk)], args.leg_timeout)
     if extra_ok and not args.no_serve:
-        out["served"] = served_run(args.rate, args.duration, "mixed", max_atoms=args.max_atoms or 300000,
-                                   max_steps=args.max_steps, mode=args.ens_mode, streams=args.streams,
-                                   chunk=args.chunk, plain=not args.no_plain_serve)
+        out["served"] = child_leg(["--workload", "serve", "--rate", str(args.rate), "--duration",
+                                   str(args.duration), "--max-steps", str(args.max_steps), "--chunk",
+                                   str(args.chunk)] + (["--no-plain-serve"] if args.no_plain_serve else []),
+                                  args.leg_timeout)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
@@ -489,6 +508,26 @@ def main():
                                                     f"({ser_time:.1f} s)"},
                                "nproc": os.cpu_count(), "affinity_cpus": aff, "cpu_model": cpu_model()}
     print(json.dumps(out))
+
+
+def child_leg(argv, timeout):
+    """one workload of the line (ensemble, served stream) in a child process of
+    its own (fresh device state; its progress lines go to this stderr); its
+    measured object, or the reason it has none"""
+    import subprocess
+
+    cmd = [sys.executable, "-u", os.path.abspath(__file__)] + argv
+    log("leg: " + " ".join(argv))
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"no result within {timeout} s", "command": " ".join(argv)}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit status {r.returncode}", "command": " ".join(argv)}
+    leg = json.loads(lines[-1]).get("leg", {})
+    leg["command"] = "python bench.py " + " ".join(argv)
+    return leg
 
 
 def traffic_key(workload, precision, skin):

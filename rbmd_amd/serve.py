@@ -6,8 +6,8 @@ the packed and served modes the systems of one kind (salt or water: the same
 force field, constraints and thermostat) form a pack (rbmd_pack_*): every
 kernel of the MD step is one launch over the blocks of all of them, replayed
 from one captured graph, so a step of many small systems costs one chain of
-launches instead of one chain each.  Each kind's pack runs on its own context
-(HIP stream), so the kinds overlap on the device.  The older mode
+launches instead of one chain each.  Each kind's pack has its own context
+(HIP stream); the kinds' rounds run one after the other (step_group).  The older mode
 ("streams") binds each system to one of K contexts and replays its own step
 graph; it is kept for comparison.  The plain loop is the reference's way of
 running many systems: one at a time, one stream, steps enqueued kernel by
@@ -192,9 +192,20 @@ class PackGroup:
         self._drop_pack()
 
 
+def step_group(gr: "PackGroup", chunk: int) -> list:
+    """One round of one kind's pack: launch it and wait for it before the next
+    kind's round is launched, so at most one pack has work in flight.  Two
+    256-system packs in flight together (one per kind, each on its own
+    context) stopped making progress once in a full benchmark process, and
+    no cause was found in this code; rounds of one pack at a time are the
+    schedule the full-size ensemble test runs."""
+    gr.launch(chunk)
+    return gr.finish()
+
+
 def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: int = 0,
                systems: Optional[list] = None, precision: str = "fp64", mode: str = "pack",
-               inputs: Optional[list] = None) -> dict:
+               inputs: Optional[list] = None, progress=None) -> dict:
     """All systems resident at once.  mode "pack": one pack per kind, each on
     its own context, `chunk` graph-replayed packed steps per round; mode
     "streams": systems spread over `streams` contexts, `chunk` steps of each
@@ -212,14 +223,18 @@ def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: 
             g.md_init()
             built.append(g)
             groups[sp.kind].add(i, g, sp.steps)
+            if progress is not None and (i + 1) % 128 == 0:
+                progress(f"{i + 1} systems set up")
         for c in ctxs.values():
             c.synchronize()
         t0 = time.perf_counter()
+        rounds = 0
         while any(gr.members for gr in groups.values()):
-            for gr in groups.values():
-                gr.launch(chunk)
-            for gr in groups.values():
-                gr.finish()
+            for gr in groups.values():  # one kind's round at a time (see step_group)
+                step_group(gr, chunk)
+            rounds += 1
+            if progress is not None and rounds % 2 == 0:
+                progress(f"{rounds} rounds of {chunk} steps")
         seconds = time.perf_counter() - t0
         for gr in groups.values():
             gr.close()
@@ -272,9 +287,9 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
     is admitted when a slot (max_live) and HBM (footprint estimate against
     hbm_budget) are free; admitted jobs are stepped `chunk` steps at a time,
     interleaved with every other live job (mode "pack": in the pack of their
-    kind, rebuilt as jobs join and leave; mode "streams": each on one of
-    `streams` contexts with its own step graph), and finish after their step
-    count.  Latency = finish - arrival.  Returns p50 / p99 latency and the
+    kind, rebuilt as jobs join and leave, one kind's round at a time; mode
+    "streams": each on one of `streams` contexts with its own step graph),
+    and finish after their step count.  Latency = finish - arrival.  Returns p50 / p99 latency and the
     aggregate atom-steps/s of the run.  `prebuilt` maps a job index to its
     (System, RunConfig), i.e. the job's input as it arrives; without it
     inputs are generated at admission.
@@ -340,10 +355,8 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
                 time.sleep(max(0.0, min(0.01, jobs[pending[nxt]].arrival * time_scale - now)))
             continue
         if pool is None:
-            for gr in groups.values():
-                gr.launch(chunk)
-            for gr in groups.values():
-                for key, _, _ in gr.finish():
+            for gr in groups.values():  # one kind's round at a time (see step_groups)
+                for key, _, _ in step_group(gr, chunk):
                     finish_job(key)
         else:
             for j, e in live.items():
