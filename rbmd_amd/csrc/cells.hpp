@@ -11,6 +11,8 @@ constexpr int kColWaves = 9;        // stencil columns of the VERLET list build
 constexpr int kScanScatterCells = 2048;  // most cells for the one-launch scan + scatter
 constexpr int kNbrWaves = 3;        // columns (wavefronts) per list-build workgroup
 constexpr int kNbrParts = kColWaves / kNbrWaves;  // list-build workgroups per work item
+constexpr int kStageRounds = 2;    // 64-candidate rounds staged per list-build batch
+constexpr int kStageCap = kStageRounds * 64;  // staged candidates per wavefront (one batch)
 
 // InWhichCell (ExecPointLocator.h:271-274): unclamped trunc
 __device__ __forceinline__ int cell_axis(double x, double lo, double d) {
@@ -23,6 +25,16 @@ __device__ __forceinline__ int wrap_axis(int v, int dims) { return (v + dims) % 
 
 __device__ __forceinline__ int pack_entry(int j, int sa, int sb, int sc) {
   return (j << kShiftBits) | ((sa + 3) | ((sb + 3) << 3) | ((sc + 3) << 6));
+}
+
+// a packed list entry's sorted index and image offset (the offset the build
+// applied, bit for bit: the same integer multiple of the cell width)
+__device__ __forceinline__ void decode(const SysView& s, int e, int& j, double& ox, double& oy, double& oz) {
+  j = e >> kShiftBits;
+  int code = e & ((1 << kShiftBits) - 1);
+  ox = s.dxdydz[0] * (double)(((code & 7) - 3) * s.dims[0]);
+  oy = s.dxdydz[1] * (double)((((code >> 3) & 7) - 3) * s.dims[1]);
+  oz = s.dxdydz[2] * (double)((((code >> 6) & 7) - 3) * s.dims[2]);
 }
 
 // wavefront-local LDS ordering (writes of all lanes visible before reads)
@@ -124,6 +136,21 @@ __device__ __forceinline__ void column_candidate(const SysView& s, int t, int n0
   pj = make_double4(p.x - ox, p.y - oy, p.z - oz, p.w);
   entry = (j << kShiftBits) | code;
 }
+
+// where candidate t of a column stream lives, without loading it: its sorted
+// index, packed entry and z image offset (x and y are the column's), so a
+// caller can issue several loads back to back before using any of them
+__device__ __forceinline__ int column_locate(int t, int n0, int n01, int jb0, int jb1, int jb2, int c0, int c1,
+                                             int c2, double oz0, double oz1, double oz2, int& entry, double& oz) {
+  const bool q1 = t >= n0, q2 = t >= n01;
+  const int j = t + (q2 ? jb2 : (q1 ? jb1 : jb0));
+  oz = q2 ? oz2 : (q1 ? oz1 : oz0);
+  entry = (j << kShiftBits) | (q2 ? c2 : (q1 ? c1 : c0));
+  return j;
+}
+
+#define COLUMN_LOCATE(t, entry, oz) \
+  column_locate(t, col.n0, col.n01, col.jb0, col.jb1, col.jb2, col.c0, col.c1, col.c2, col.oz0, col.oz1, col.oz2, entry, oz)
 
 #define COLUMN_CANDIDATE(t, pj, entry)                                                                             \
   column_candidate(s, t, col.n0, col.n01, col.jb0, col.jb1, col.jb2, col.c0, col.c1, col.c2, col.ox, col.oy, \

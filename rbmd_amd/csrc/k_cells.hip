@@ -421,7 +421,8 @@ __device__ void misfit_block(const SysView& s, int wave, int lane, int slot) {
 // One workgroup: kNbrWaves columns of one work item (or of one misfit slot);
 // block bid is part bid % kNbrParts of work item bid / kNbrParts.
 __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const int bid) {
-  __shared__ double4 stage[kNbrWaves][kWave];  // position, entry in the low word of w
+  __shared__ float4 stage[kNbrWaves][kStageCap];  // FP32 position from the chunk's box corner, entry bits in w
+  __shared__ double4 stage64[kNbrWaves][kStageCap];  // the same candidates' shifted FP64 positions (re-tests only)
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = bid / kNbrParts;
@@ -474,69 +475,130 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   make_column(s, ca, cb, cc, a, b, lane, col);
   const int total = col.total;
   const double cut2 = s.list_cut2;  // rc^2, or (rc + skin)^2 for a skin list
+  // FP32 test with an exact FP64 re-test near the thresholds.  Coordinates are
+  // taken from the chunk's box corner bl: every staged candidate lies within
+  // the list cutoff of the box, so each coordinate is below 3 rc in size and
+  // carries at most 2^-24 * 3 rc of conversion error; with the FP32 difference
+  // and sum of squares, |d2_f32 - d2_f64| < 1.7e-6 rc^2 where d2 is near rc^2.
+  // Candidates within 1e-5 rc^2 of rc^2 - 1e-4, or below 1e-3, are decided by
+  // the FP64 predicate of the per-candidate path (the reference's
+  // rc^2 - d^2 > 1e-4 && d^2 > 1e-4 on the same rounded vector), all others by
+  // the FP32 comparison, which then agrees with it.  The atom itself (same
+  // cell, no image shift) is rejected by its entry: its d^2 is exactly 0.
+  const float cut2m_f = (float)(cut2 - 0.0001);
+  const float band_f = (float)(1e-5 * cut2);
+  const float lo_f = 1e-3f;
+  const float lix = (float)(pi.x - bl[0]), liy = (float)(pi.y - bl[1]), liz = (float)(pi.z - bl[2]);
+  const int self_entry = active ? pack_entry(i, 0, 0, 0) : -1;
+  // Accepted entries collect in four registers and leave as one 16-byte store
+  // per four entries (a segment starts on a 64-byte boundary and holds a
+  // multiple of 16 slots), so the row is written in whole 16-byte pieces
+  // rather than one scattered 4-byte store per entry.
   int cnt = 0;
-  // software pipeline: the candidate of the next round is loaded ahead
-  double4 nxt = make_double4(0, 0, 0, 0);
-  int nxt_e = 0;
-  if (lane < total) COLUMN_CANDIDATE(lane, nxt, nxt_e);
-  for (int base = 0; base < total; base += kWave) {
-    const int t = base + lane;
-    const double4 pj = nxt;
-    const int entry = nxt_e;
-    if (t + kWave < total) COLUMN_CANDIDATE(t + kWave, nxt, nxt_e);
-    bool keep = false;
-    if (t < total) {
-      double dx = fmax(0.0, fmax(bl[0] - pj.x, pj.x - bh[0]));
-      double dy = fmax(0.0, fmax(bl[1] - pj.y, pj.y - bh[1]));
-      double dz = fmax(0.0, fmax(bl[2] - pj.z, pj.z - bh[2]));
-      keep = dx * dx + dy * dy + dz * dz < cut2;  // superset of every lane's acceptance
+  int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+  auto push = [&](int e) {
+    const int k = cnt & 3;
+    b0 = k == 0 ? e : b0;
+    b1 = k == 1 ? e : b1;
+    b2 = k == 2 ? e : b2;
+    b3 = k == 3 ? e : b3;
+    if (k == 3 && cnt < seg_cap) *reinterpret_cast<int4*>(row + (cnt - 3)) = make_int4(b0, b1, b2, b3);
+    ++cnt;
+  };
+  // the FP32 predicate of one staged candidate; nr: decide it in FP64
+  auto test = [&](const float4 q, bool& ok, bool& nr) {
+    const float rx = q.x - lix, ry = q.y - liy, rz = q.z - liz;
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    const bool live = active & (__float_as_int(q.w) != self_entry);
+    ok = live & (d2 < cut2m_f) & (d2 > lo_f);
+    nr = live & ((fabsf(d2 - cut2m_f) < band_f) | (d2 <= lo_f));
+  };
+  // the per-candidate FP64 predicate on the build's own vector (the staged
+  // shifted position; no global load in the test loop, where a load would
+  // make every iteration wait for the row stores before it)
+  auto exact = [&](int slot) -> bool {
+    const double4 q = stage64[w][slot];
+    const double rx = q.x - pi.x;
+    const double ry = q.y - pi.y;
+    const double rz = q.z - pi.z;
+    const double d2 = dist2(rx, ry, rz);
+    return (cut2 - d2 > 0.0001) & (d2 > 0.0001);
+  };
+  // Batches of up to kStageRounds x 64 candidates: the batch's loads are
+  // issued back to back, the survivors of the box cull are staged in stream
+  // order, then every lane tests them all.  Loads and the entry stores are
+  // counted by one in-order counter, so waiting for a load also waits for
+  // every store issued before it: with whole batches there is one such wait
+  // per batch, not one per 64 candidates.
+  for (int base = 0; base < total; base += kStageRounds * kWave) {
+    double4 raw[kStageRounds];
+    double ozr[kStageRounds];
+    int er[kStageRounds];
+#pragma unroll
+    for (int r = 0; r < kStageRounds; ++r) {  // straight-line loads (past the end: the last candidate again)
+      const int t = base + r * kWave + lane;
+      int e;
+      raw[r] = s.spos[COLUMN_LOCATE(t < total ? t : total - 1, e, ozr[r])];
+      er[r] = t < total ? e : -1;
     }
-    const unsigned long long bal = __ballot(keep);
-    const int kc = __popcll(bal);
+    int kc = 0;
+#pragma unroll
+    for (int r = 0; r < kStageRounds; ++r) {
+      bool keep = false;
+      double4 pj = make_double4(0, 0, 0, 0);
+      if (er[r] >= 0) {
+        pj = make_double4(raw[r].x - col.ox, raw[r].y - col.oy, raw[r].z - ozr[r], raw[r].w);
+        double dx = fmax(0.0, fmax(bl[0] - pj.x, pj.x - bh[0]));
+        double dy = fmax(0.0, fmax(bl[1] - pj.y, pj.y - bh[1]));
+        double dz = fmax(0.0, fmax(bl[2] - pj.z, pj.z - bh[2]));
+        keep = dx * dx + dy * dy + dz * dz < cut2;  // superset of every lane's acceptance
+      }
+      const unsigned long long bal = __ballot(keep);
+      if (keep) {
+        const int slot = kc + __popcll(bal & ((1ull << lane) - 1ull));
+        stage[w][slot] = make_float4((float)(pj.x - bl[0]), (float)(pj.y - bl[1]), (float)(pj.z - bl[2]),
+                                     __int_as_float(er[r]));
+        stage64[w][slot] = pj;
+      }
+      kc += __popcll(bal);
+    }
     if (kc == 0) continue;
-    if (keep) {
-      int slot = __popcll(bal & ((1ull << lane) - 1ull));
-      stage[w][slot] = make_double4(pj.x, pj.y, pj.z, __hiloint2double(0, entry));
-    }
     wave_sync();
-    // test every staged candidate; an accepting lane appends the candidate's
-    // entry (an LDS broadcast) to its own row, so entries stay in stream
-    // order.  Four candidates are read at once and their predicates computed
-    // before any store, so one LDS round trip covers four candidates.
+    // every staged candidate is tested by every lane; an accepting lane
+    // appends the candidate's entry to its own row in stream order.  Four
+    // candidates are read per LDS round trip.
     int u = 0;
     for (; u + 4 <= kc; u += 4) {
-      double4 q[4];
+      float4 q[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) q[r] = stage[w][u + r];
-      bool ok[4];
+      bool ok[4], nr[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double rx = q[r].x - pi.x;
-        const double ry = q[r].y - pi.y;
-        const double rz = q[r].z - pi.z;
-        const double d2 = dist2(rx, ry, rz);
-        ok[r] = active & (cut2 - d2 > 0.0001) & (d2 > 0.0001);
+      for (int r = 0; r < 4; ++r) test(q[r], ok[r], nr[r]);
+      if (__ballot(nr[0] | nr[1] | nr[2] | nr[3]) != 0ull) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (nr[r]) ok[r] = exact(u + r);
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (ok[r]) {
-          if (cnt < seg_cap) row[cnt] = __double2loint(q[r].w);
-          ++cnt;
-        }
-      }
+      for (int r = 0; r < 4; ++r)
+        if (ok[r]) push(__float_as_int(q[r].w));
     }
     for (; u < kc; ++u) {
-      const double4 q = stage[w][u];
-      const double rx = q.x - pi.x;
-      const double ry = q.y - pi.y;
-      const double rz = q.z - pi.z;
-      const double d2 = dist2(rx, ry, rz);
-      if (active & (cut2 - d2 > 0.0001) & (d2 > 0.0001)) {
-        if (cnt < seg_cap) row[cnt] = __double2loint(q.w);
-        ++cnt;
-      }
+      const float4 q = stage[w][u];
+      bool ok, nr;
+      test(q, ok, nr);
+      if (nr) ok = exact(u);
+      if (ok) push(__float_as_int(q.w));
     }
     wave_sync();
+  }
+  {  // the last (cnt & 3) entries
+    const int k = cnt & 3;
+    const int first = cnt - k;
+    if (k > 0 && first < seg_cap) row[first] = b0;
+    if (k > 1 && first + 1 < seg_cap) row[first + 1] = b1;
+    if (k > 2 && first + 2 < seg_cap) row[first + 2] = b2;
   }
   if (cnt > seg_cap) atomicOr(s.status, 4);
   if (active) s.nbr_num[wave * s.n + i] = cnt < seg_cap ? cnt : seg_cap;
