@@ -400,6 +400,7 @@ rbmd_status use_geom(rbmd_sys* s, bool skin) {
   for (int d = 0; d < 3; ++d) {
     s->v.dims[d] = g.dims[d];
     s->v.dxdydz[d] = g.dxdydz[d];
+    s->v.shift_len[d] = g.dxdydz[d] * (double)g.dims[d];
   }
   s->v.ncell = g.ncell;
   s->v.n_items = g.n_items;
@@ -869,6 +870,7 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   }
   v.volume = v.box[0] * v.box[1] * v.box[2];
   geometry(*p, d->lo, d->hi, v.dims, &v.nc, v.dxdydz);
+  for (int k = 0; k < 3; ++k) v.shift_len[k] = v.dxdydz[k] * (double)v.dims[k];
   v.ncell = v.dims[0] * v.dims[1] * v.dims[2];
   v.cut = p->cut_off;
   v.cut2 = p->cut_off * p->cut_off;

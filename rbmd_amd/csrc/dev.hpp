@@ -61,6 +61,9 @@ struct SysView {
   int use_min_image;  // some box edge < 2 rc: the pair vector may need the minimum image
   int mixed;        // RBMD_PREC_MIXED: the pair scalar in FP32 (pair vectors and sums stay FP64)
   double lo[3], hi[3], box[3], dxdydz[3];
+  // dxdydz[k] * dims[k]: one image shift along axis k as the cell code forms
+  // it (kept beside dims and dxdydz whenever they change)
+  double shift_len[3];
   double volume, cut, cut2, rs, rs2, alpha, sqrt_alpha, qqr2e;
   // per atom, original order (SoA)
   double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz;

@@ -150,7 +150,7 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
         const int tj = h == 0 ? ta : tb;
         int j;
         double ox, oy, oz;
-        decode(s, e, j, ox, oy, oz);  // the build's image offset, bit for bit
+        decode1(s, e, j, ox, oy, oz);  // the build's image offset, bit for bit (stencil half width 1)
         double rx = (p.x - ox) - pi.x;
         double ry = (p.y - oy) - pi.y;
         double rz = (p.z - oz) - pi.z;

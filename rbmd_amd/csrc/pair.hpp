@@ -2,6 +2,7 @@
 // neighbour-list pair kernels and the fused cell-stencil force kernel.
 #pragma once
 #include "dev.hpp"
+#include "cells.hpp"
 
 namespace rbmd {
 
@@ -37,12 +38,16 @@ __device__ __forceinline__ void min_image(const SysView& s, double& rx, double& 
   if (fabs(rz) > s.box[2] * 0.5) rz -= (rz > 0 ? s.box[2] : -s.box[2]);
 }
 
-__device__ __forceinline__ void decode(const SysView& s, int e, int& j, double& ox, double& oy, double& oz) {
+
+// decode() for lists over cells at least rc wide (VERLET, stencil half width
+// 1): every image shift is -1, 0 or +1 box lengths, so the offset is the shift
+// times shift_len, which equals decode()'s dxdydz * (shift * dims) bit for bit
+// (a product with -1, 0 or 1 is exact) without its integer multiplies.
+__device__ __forceinline__ void decode1(const SysView& s, int e, int& j, double& ox, double& oy, double& oz) {
   j = e >> kShiftBits;
-  int code = e & ((1 << kShiftBits) - 1);
-  ox = s.dxdydz[0] * (double)(((code & 7) - 3) * s.dims[0]);
-  oy = s.dxdydz[1] * (double)((((code >> 3) & 7) - 3) * s.dims[1]);
-  oz = s.dxdydz[2] * (double)((((code >> 6) & 7) - 3) * s.dims[2]);
+  ox = (double)((e & 7) - 3) * s.shift_len[0];
+  oy = (double)(((e >> 3) & 7) - 3) * s.shift_len[1];
+  oz = (double)(((e >> 6) & 7) - 3) * s.shift_len[2];
 }
 
 __device__ __forceinline__ void vir_add(double* v, double rx, double ry, double rz, double fx, double fy, double fz,
