@@ -274,7 +274,7 @@ def main():
                          "per-system step graphs spread over --streams contexts")
     ap.add_argument("--streams", type=int, default=8, help="ensemble / serve with --ens-mode streams: contexts")
     ap.add_argument("--chunk", type=int, default=50, help="ensemble / serve: graph steps per enqueue round")
-    ap.add_argument("--rate", type=float, default=0.8, help="serve: Poisson arrivals per second")
+    ap.add_argument("--rate", type=float, default=1.5, help="serve: Poisson arrivals per second")
     ap.add_argument("--duration", type=float, default=30.0, help="serve: seconds of arrivals")
     ap.add_argument("--max-atoms", type=int, default=None,
                     help="ensemble / serve: largest system (default 30,000 / 300,000)")

@@ -6,8 +6,9 @@ the packed and served modes the systems of one kind (salt or water: the same
 force field, constraints and thermostat) form a pack (rbmd_pack_*): every
 kernel of the MD step is one launch over the blocks of all of them, replayed
 from one captured graph, so a step of many small systems costs one chain of
-launches instead of one chain each.  Each kind's pack has its own context
-(HIP stream); the kinds' rounds run one after the other (step_group).  The older mode
+launches instead of one chain each.  Each kind's packs have their own context
+(HIP stream); rounds run one pack at a time (step_group; the served loop picks
+each round's pack by remaining work).  The older mode
 ("streams") binds each system to one of K contexts and replays its own step
 graph; it is kept for comparison.  The plain loop is the reference's way of
 running many systems: one at a time, one stream, steps enqueued kernel by
@@ -17,6 +18,7 @@ The reference has no counterpart for this module (its executioner runs one
 system, framework/executioner/Executioner.cpp:62-129); each system's step is
 the same ExecutionNVT::Solve sequence as rbmd_md_step, with the same bits.
 """
+import collections
 import dataclasses
 import math
 import time
@@ -280,19 +282,25 @@ def run_packed(specs: List[JobSpec], streams: int = 8, chunk: int = 50, device: 
 
 def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: float = 200e9, max_live: int = 64,
           device: int = 0, time_scale: float = 1.0, results: Optional[dict] = None,
-          prebuilt: Optional[dict] = None, precision: str = "fp64", mode: str = "pack", progress=None) -> dict:
+          prebuilt: Optional[dict] = None, precision: str = "fp64", mode: str = "pack", progress=None,
+          small_atoms: int = 20000, max_packs: int = 16, age_rate: float = 1.5e8) -> dict:
     """Continuous batching of a job stream.
 
     Jobs arrive at spec.arrival * time_scale seconds after the start.  A job
     is admitted when a slot (max_live) and HBM (footprint estimate against
-    hbm_budget) are free; admitted jobs are stepped `chunk` steps at a time,
-    interleaved with every other live job (mode "pack": in the pack of their
-    kind, rebuilt as jobs join and leave, one kind's round at a time; mode
-    "streams": each on one of `streams` contexts with its own step graph),
-    and finish after their step count.  Latency = finish - arrival.  Returns p50 / p99 latency and the
-    aggregate atom-steps/s of the run.  `prebuilt` maps a job index to its
-    (System, RunConfig), i.e. the job's input as it arrives; without it
-    inputs are generated at admission.
+    hbm_budget) are free, and finishes after its step count.  Mode "pack":
+    each round runs, for up to `chunk` steps, the live job with the least
+    remaining work (atoms x steps left, less age_rate atom-steps per second
+    it has waited, so a large job is not starved by a stream of small ones),
+    in one pack with the live jobs of its kind that are at most twice its
+    size (or up to `small_atoms`): a pack steps about as fast as its
+    smallest member, and small jobs are not held to a large job's step time.
+    Packs are kept while their members are live; a job alone steps with its
+    own launches.  Mode "streams": every live job is stepped `chunk` steps per round
+    on one of `streams` contexts with its own step graph.  Latency = finish -
+    arrival.  Returns p50 / p99 latency and the aggregate atom-steps/s of the
+    run.  `prebuilt` maps a job index to its (System, RunConfig), i.e. the
+    job's input as it arrives; without it inputs are generated at admission.
     """
     if mode not in ("pack", "streams"):
         raise ValueError(mode)
@@ -302,10 +310,10 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
     used = 0.0
     lat = []
     done_atom_steps = 0
+    packs = collections.OrderedDict()  # (kind, member jobs) -> Pack, while every member is live
     if mode == "pack":
         kinds = sorted({j.kind for j in jobs})
         ctxs = {k: Context(device) for k in kinds}
-        groups = {k: PackGroup(ctxs[k]) for k in kinds}
         pool = None
     else:
         pool = Pool(streams, device)
@@ -314,6 +322,8 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
 
     def finish_job(j):
         nonlocal used, done_atom_steps
+        for key in [key for key in packs if j in key[1]]:  # a pack must not outlive its systems
+            packs.pop(key).close()
         g, _, k, need = live.pop(j)
         lat.append((time.perf_counter() - t_start) - jobs[j].arrival * time_scale)
         done_atom_steps += jobs[j].atoms * jobs[j].steps
@@ -348,16 +358,40 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
             g.md_init()
             live[j] = [g, 0, k, need]
             used += need
-            if pool is None:
-                groups[k].add(j, g, jobs[j].steps)
         if not live:
             if nxt < len(pending):
                 time.sleep(max(0.0, min(0.01, jobs[pending[nxt]].arrival * time_scale - now)))
             continue
         if pool is None:
-            for gr in groups.values():  # one kind's round at a time (see step_groups)
-                for key, _, _ in step_group(gr, chunk):
-                    finish_job(key)
+            t_now = time.perf_counter() - t_start
+
+            def urgency(j):  # remaining work, less what the wait so far is worth
+                left = (jobs[j].steps - live[j][1]) * jobs[j].atoms
+                return (left - age_rate * (t_now - jobs[j].arrival * time_scale), jobs[j].arrival)
+
+            head = min(live, key=urgency)
+            kind = jobs[head].kind
+            limit = max(2 * jobs[head].atoms, small_atoms)
+            members = tuple(sorted(j for j in live if jobs[j].kind == kind and jobs[j].atoms <= limit))
+            n = min(chunk, min(jobs[j].steps - live[j][1] for j in members))
+            if len(members) == 1:  # one system: its own launches (a replayed graph is no faster there)
+                live[head][0].md_step(n)
+                ctxs[kind].synchronize()
+            else:
+                key = (kind, members)
+                if key in packs:
+                    packs.move_to_end(key)
+                else:
+                    packs[key] = Pack(ctxs[kind], [live[j][0] for j in members])
+                    while len(packs) > max_packs:  # the least recently used pack goes
+                        packs.popitem(last=False)[1].close()
+                packs[key].enqueue(n)
+                packs[key].wait()
+            for j in members:
+                live[j][1] += n
+            for j in members:
+                if live[j][1] >= jobs[j].steps:
+                    finish_job(j)
         else:
             for j, e in live.items():
                 n = min(chunk, jobs[j].steps - e[1])
@@ -371,11 +405,11 @@ def serve(jobs: List[JobSpec], streams: int = 8, chunk: int = 50, hbm_budget: fl
             for j in finished:
                 finish_job(j)
     wall = time.perf_counter() - t_start
+    for pk in packs.values():
+        pk.close()
     if pool is not None:
         pool.close()
     else:
-        for gr in groups.values():
-            gr.close()
         for c in ctxs.values():
             c.close()
     lat = np.sort(np.asarray(lat))
