@@ -1729,22 +1729,45 @@ static bool kring_on(const rbmd_sys* s) {
 // the reported energies; without them it computes the same forces and the
 // kinetic energy the thermostat needs, and the thermo state it leaves holds
 // no valid pressure or energies (the last step of every call has thermo on).
-static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool thermo) {
+// true when the thermostat's step-start phase can run in the previous step's
+// end-of-step launch (phase 6): Nose-Hoover and plain velocity Verlet
+static bool nhc_merges(const rbmd_sys* s) {
+  return s->p.temp_ctrl == RBMD_TEMP_NOSE_HOOVER || s->p.temp_ctrl == RBMD_TEMP_NONE;
+}
+
+// the thermostat's step-start launch alone (the first step of a call)
+static void enqueue_step_start(rbmd_sys* s) {
+  rbmd_ctx* c = s->ctx;
+  c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, false), nhc_merges(s) ? 1 : 5, c->stream); });
+}
+
+// One MD step.  start: the step begins with its own thermostat launch (else
+// the previous step's end ran it); merge_end: the end-of-step thermostat
+// launch also starts the next step (every step of a call but the last).
+static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool thermo, bool start, bool merge_end) {
   rbmd_ctx* c = s->ctx;
   hipStream_t st = c->stream;
   const int tc = s->p.temp_ctrl;
   const bool nh = tc == RBMD_TEMP_NOSE_HOOVER || tc == RBMD_TEMP_NONE;
   const bool shake = shake_on(s);
-  c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, false), nh ? 1 : 5, st); });
+  if (start || !nh) enqueue_step_start(s);
   if (from_ring)  // the nhc phase above advanced the device step counter
     c->run("kvec_slot", [&] { launch_kvec_slot(s->kgraph, s->kring, s->ring_R, 3 * s->P, &s->nhc->step, st); });
   const double* scale = (tc == RBMD_TEMP_NOSE_HOOVER) ? &s->nhc->factor : nullptr;
   // also bins (per-step lists: the reference's locator bins before SHAKE)
-  c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, s->skin_active ? 0 : 1); });
-  if (shake) {
-    c->run("shake", [&] { launch_shake(s->v, s->iv, st, s->seg_rows(SEG_SHAKE), s->shake_iter); });
+  const int do_bin = s->skin_active ? 0 : 1;
+  if (shake && s->nfree == 0) {  // every atom constrained: one launch for both
+    c->run("first_half_shake", [&] {
+      launch_first_half_shake(s->v, s->iv, st, scale, do_bin, s->seg_rows(SEG_SHAKE), s->shake_iter);
+    });
     s->seg_used[SEG_SHAKE] = (s->nmol + kBlock - 1) / kBlock;
-    if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 1);
+  } else {
+    c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, do_bin); });
+    if (shake) {
+      c->run("shake", [&] { launch_shake(s->v, s->iv, st, s->seg_rows(SEG_SHAKE), s->shake_iter); });
+      s->seg_used[SEG_SHAKE] = (s->nmol + kBlock - 1) / kBlock;
+      if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 1);
+    }
   }
   // skin list: the displacement test on the final positions, which also bins
   // them for a rebuild; the cell and list kernels below run in full only on
@@ -1774,9 +1797,11 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool the
     launch_second_half(s->v, s->iv, st, 1, has_k(s), has_bonded(s), s->seg_rows(SEG_KE), &s->seg_used[SEG_KE]);
   });
   if (nh) {
-    c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 2, st, thermo ? 0 : 1); });
+    c->run("nhc", [&] {
+      launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), merge_end ? 6 : 2, st, thermo ? 0 : 1);
+    });
     if (tc == RBMD_TEMP_NOSE_HOOVER && !(shake && s->nfree == 0))
-      c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
+      c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor_end); });
     if (shake) {
       c->run("rattle", [&] { launch_rattle(s->v, s->iv, st); });
       if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 0);
@@ -1788,15 +1813,15 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool the
       c->run("ke_rows", [&] { launch_ke_rows(s->v, s->iv, st, s->seg_rows(SEG_KE)); });
     }
     c->run("nhc", [&] { launch_nhc_rows(s->nhc, s->np, rows_desc(s, true), 4, st, thermo ? 0 : 1); });
-    c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor); });
+    c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor_end); });
   }
 }
 
-static rbmd_status md_one_step(rbmd_sys* s, bool thermo) {
+static rbmd_status md_one_step(rbmd_sys* s, bool first, bool last) {
   const int64_t step = s->step + 1;
   const double* kv = nullptr;
   RC_TRY(kvec_for_step(s, step, &kv));
-  enqueue_step(s, kv, false, thermo);
+  enqueue_step(s, kv, false, last, first, nhc_merges(s) && !last);
   s->step = step;
   return RBMD_OK;
 }
@@ -1818,7 +1843,10 @@ static rbmd_status capture_step(rbmd_sys* s) {
     const bool prof = c->prof;
     c->prof = false;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-    enqueue_step(s, ring ? s->kgraph : s->kcur, ring, thermo != 0);
+    // graph 0: a step between others of its call; graph 1: a call's last
+    // step.  Neither starts with the thermostat launch: the previous step's
+    // end (or a launch before the first replay of the call) ran it.
+    enqueue_step(s, ring ? s->kgraph : s->kcur, ring, thermo != 0, !nhc_merges(s), nhc_merges(s) && !thermo);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
     c->prof = prof;
@@ -1836,7 +1864,7 @@ rbmd_status rbmd_md_step(rbmd_sys* s, int64_t n_steps) {
   if (!s->initialized) return fail(RBMD_ERR_STATE, "rbmd_md_init has not run");
   RC_TRY(prepare_skin(s));
   for (int64_t k = 0; k < n_steps; ++k) {
-    RC_TRY(md_one_step(s, k == n_steps - 1));
+    RC_TRY(md_one_step(s, k == 0, k == n_steps - 1));
     if (s->p.energy_every > 0 && has_k(s) && s->step % s->p.energy_every == 0) {
       RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
       s->energy_step = s->step;
@@ -1857,6 +1885,7 @@ static rbmd_status enqueue_graph_steps(rbmd_sys* s, int64_t n_steps) {
   if (!s->graph_ok) RC_TRY(capture_step(s));
   hipStream_t st = s->ctx->stream;
   const bool ring = kring_on(s);
+  if (n_steps > 0 && nhc_merges(s)) enqueue_step_start(s);  // the graphs start after it
   for (int64_t k = 0; k < n_steps; ++k) {
     const int64_t step = s->step + 1;
     if (ring) {

@@ -263,6 +263,10 @@ struct IntView {
 };
 void launch_first_half(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin);
 void launch_shake(const SysView& s, const IntView& iv, hipStream_t st, double* partials, int* max_iter);
+// launch_first_half + launch_shake in one launch, for systems whose every atom
+// is in a SHAKE triplet (bit for bit the two)
+void launch_first_half_shake(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin,
+                             double* partials, int* max_iter);
 void launch_rattle(const SysView& s, const IntView& iv, hipStream_t st);
 void launch_bonded(const SysView& s, const IntView& iv, hipStream_t st, int thermo, double* partials, int* nblocks);
 void launch_second_half(const SysView& s, const IntView& iv, hipStream_t st, int kick, int has_k, int has_b,
@@ -276,10 +280,11 @@ struct NhcState {
   double eta[3], eta_dot[4], eta_dotdot[3], eta_mass[3];
   double t_target, ke_target;
   double tempT, tempT_sum, pressure;
-  double factor;         // last velocity scale
+  double factor;         // velocity scale of the step start (phase 1), or the last phase's
   double virial[6];
   double e_lj, e_coul, e_bond, e_angle;
   int64_t step;
+  double factor_end;     // velocity scale of the step end (phases 2, 3, 4)
 };
 struct NhcParams {
   int tdof, mode;        // mode: 0 NHC, 1 Berendsen, 2 rescale, 3 none
@@ -293,7 +298,8 @@ struct NhcParams {
   int* shake_iter;       // SHAKE iteration maximum of the step, zeroed at the step start (phases 1, 5)
 };
 // phase 1: step++ and NHC half step (InitialIntegrate); 2: ComputeTempe + pressure + NHC
-// (FinalIntegrate); 3: SetUp; 4: Berendsen/rescale end of step; 5: step++ only
+// (FinalIntegrate); 3: SetUp; 4: Berendsen/rescale end of step; 5: step++ only;
+// 6: phase 2 then phase 1 (between two steps of one call)
 // ke_only: reduce the kinetic-energy segment only (steps without thermo sums)
 void launch_nhc_rows(NhcState* st_dev, const NhcParams& p, const RowsDesc& rd, int phase, hipStream_t st,
                      int ke_only = 0);

@@ -15,58 +15,64 @@
 
 namespace rbmd {
 
+// NHC scale (optional), kick, drift, PBC of atom i; returns its cell when
+// do_bin (the binning of the drifted position), else -1
+__device__ __forceinline__ int first_half_atom(const SysView& s, const IntView& iv, const double* scale, int do_bin,
+                                               int i) {
+#pragma clang fp contract(off)  // first_half_regs rounds the same way
+  double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+  if (scale) {
+    double f = *scale;
+    vx = f * vx;
+    vy = f * vy;
+    vz = f * vz;
+  }
+  s.ovx[i] = vx;
+  s.ovy[i] = vy;
+  s.ovz[i] = vz;
+  double m = s.mass[i];
+  vx += 0.5 * s.fx[i] / m * iv.dt * iv.fmt2v;
+  vy += 0.5 * s.fy[i] / m * iv.dt * iv.fmt2v;
+  vz += 0.5 * s.fz[i] / m * iv.dt * iv.fmt2v;
+  s.vx[i] = vx;
+  s.vy[i] = vy;
+  s.vz[i] = vz;
+  double p[3] = {s.x[i], s.y[i], s.z[i]};
+  s.ox[i] = p[0];
+  s.oy[i] = p[1];
+  s.oz[i] = p[2];
+  p[0] += vx * iv.dt;
+  p[1] += vy * iv.dt;
+  p[2] += vz * iv.dt;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (p[d] < s.lo[d]) {
+      p[d] += s.box[d];
+      s.img[3 * i + d] -= 1;
+    } else if (p[d] >= s.hi[d]) {
+      p[d] -= s.box[d];
+      s.img[3 * i + d] += 1;
+    }
+  }
+  s.x[i] = p[0];
+  s.y[i] = p[1];
+  s.z[i] = p[2];
+  if (!isfinite(p[0] + p[1] + p[2] + vx + vy + vz)) atomicOr(s.status, 1);
+  if (!do_bin) return -1;
+  int a = bin_axis(p[0], s.lo[0], s.dxdydz[0], s.dims[0]);
+  int b = bin_axis(p[1], s.lo[1], s.dxdydz[1], s.dims[1]);
+  int c = bin_axis(p[2], s.lo[2], s.dxdydz[2], s.dims[2]);
+  const int cell = a + b * s.dims[0] + c * s.dims[0] * s.dims[1];
+  s.cell_of[i] = cell;
+  return cell;
+}
+
 // NHC scale (optional), kick, drift, PBC, binning
 __device__ __forceinline__ void first_half_body(const SysView& s, const IntView& iv, const double* scale, int do_bin, const int bid) {
   extern __shared__ int lds_cells[];
   int i = bid * blockDim.x + threadIdx.x;
   int cell = -1;
-  if (i < s.n) {
-    double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
-    if (scale) {
-      double f = *scale;
-      vx = f * vx;
-      vy = f * vy;
-      vz = f * vz;
-    }
-    s.ovx[i] = vx;
-    s.ovy[i] = vy;
-    s.ovz[i] = vz;
-    double m = s.mass[i];
-    vx += 0.5 * s.fx[i] / m * iv.dt * iv.fmt2v;
-    vy += 0.5 * s.fy[i] / m * iv.dt * iv.fmt2v;
-    vz += 0.5 * s.fz[i] / m * iv.dt * iv.fmt2v;
-    s.vx[i] = vx;
-    s.vy[i] = vy;
-    s.vz[i] = vz;
-    double p[3] = {s.x[i], s.y[i], s.z[i]};
-    s.ox[i] = p[0];
-    s.oy[i] = p[1];
-    s.oz[i] = p[2];
-    p[0] += vx * iv.dt;
-    p[1] += vy * iv.dt;
-    p[2] += vz * iv.dt;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (p[d] < s.lo[d]) {
-        p[d] += s.box[d];
-        s.img[3 * i + d] -= 1;
-      } else if (p[d] >= s.hi[d]) {
-        p[d] -= s.box[d];
-        s.img[3 * i + d] += 1;
-      }
-    }
-    s.x[i] = p[0];
-    s.y[i] = p[1];
-    s.z[i] = p[2];
-    if (!isfinite(p[0] + p[1] + p[2] + vx + vy + vz)) atomicOr(s.status, 1);
-    if (do_bin) {
-      int a = bin_axis(p[0], s.lo[0], s.dxdydz[0], s.dims[0]);
-      int b = bin_axis(p[1], s.lo[1], s.dxdydz[1], s.dims[1]);
-      int c = bin_axis(p[2], s.lo[2], s.dxdydz[2], s.dims[2]);
-      cell = a + b * s.dims[0] + c * s.dims[0] * s.dims[1];
-      s.cell_of[i] = cell;
-    }
-  }
+  if (i < s.n) cell = first_half_atom(s, iv, scale, do_bin, i);
   if (do_bin) count_cells(s, cell, lds_cells);
 }
 
@@ -102,23 +108,16 @@ __device__ __forceinline__ void st3(double* x, double* y, double* z, int i, D3 v
 }
 
 // SHAKE on (old x, old v, F) of one H-O-H triplet; writes x, v, image flags
-__device__ __forceinline__ void shake_body(const SysView& s, const IntView& iv, double* rows, int* max_iter_out, const int bid) {
-  int a = bid * blockDim.x + threadIdx.x;
-  double vir[6] = {0, 0, 0, 0, 0, 0};
-  int my_iter = 0;
-  if (a < iv.nmol) {
+// SHAKE of one molecule from its atoms' old positions ox, old velocities ovv,
+// forces ff and masses m; writes x, v, image flags; vir: the constraint virial
+__device__ __forceinline__ void shake_mol(const SysView& s, const IntView& iv, const int (&ids)[3], const D3 (&ox)[3],
+                                          const D3 (&ovv)[3], const D3 (&ff)[3], const double (&m)[3], double (&vir)[6],
+                                          int& my_iter) {
+  {
     const double dt = iv.dt, fmt2v = iv.fmt2v;
-    int ids[3] = {iv.angle_atoms[3 * a], iv.angle_atoms[3 * a + 1], iv.angle_atoms[3 * a + 2]};
-    D3 ox[3], sp[3], ovv[3], ff[3];
-    double m[3];
-    for (int t = 0; t < 3; ++t) {
-      int id = ids[t];
-      ox[t] = ld(s.ox, s.oy, s.oz, id);
-      ovv[t] = ld(s.ovx, s.ovy, s.ovz, id);
-      ff[t] = ld(s.fx, s.fy, s.fz, id);
-      m[t] = s.mass[id];
+    D3 sp[3];
+    for (int t = 0; t < 3; ++t)
       sp[t] = add(add(ox[t], mul(dt, ovv[t])), mul(fmt2v, divs(mul(0.5 * dt * dt, ff[t]), m[t])));
-    }
     const double bond1 = 1.0, bond2 = 1.0;
     const double bond12 = sqrt(bond1 * bond1 + bond2 * bond2 - 2.0 * bond1 * bond2 * cos((109.4700 / 180.0) * kPi));
     D3 r01 = mimg(s, ox[1], ox[0]);
@@ -223,13 +222,126 @@ __device__ __forceinline__ void shake_body(const SysView& s, const IntView& iv, 
     vir[4] = l01 * r01.x * r01.z + l20 * r20.x * r20.z + l12 * r12.x * r12.z;
     vir[5] = l01 * r01.y * r01.z + l20 * r20.y * r20.z + l12 * r12.y * r12.z;
   }
+}
+
+// the SHAKE rows of a block and the iteration maximum
+__device__ __forceinline__ void shake_finish(double* rows, int* max_iter_out, const double (&vir)[6], int my_iter,
+                                             const int bid) {
   if (max_iter_out) atomicMax(max_iter_out, my_iter);
   const double vals[kRowShake] = {vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
   block_write_row<kRowShake>(vals, rows + (size_t)bid * kRowShake);
 }
 
+// SHAKE on (old x, old v, F) of one H-O-H triplet per thread
+__device__ __forceinline__ void shake_body(const SysView& s, const IntView& iv, double* rows, int* max_iter_out, const int bid) {
+  int a = bid * blockDim.x + threadIdx.x;
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  int my_iter = 0;
+  if (a < iv.nmol) {
+    int ids[3] = {iv.angle_atoms[3 * a], iv.angle_atoms[3 * a + 1], iv.angle_atoms[3 * a + 2]};
+    D3 ox[3], ovv[3], ff[3];
+    double m[3];
+    for (int t = 0; t < 3; ++t) {
+      int id = ids[t];
+      ox[t] = ld(s.ox, s.oy, s.oz, id);
+      ovv[t] = ld(s.ovx, s.ovy, s.ovz, id);
+      ff[t] = ld(s.fx, s.fy, s.fz, id);
+      m[t] = s.mass[id];
+    }
+    shake_mol(s, iv, ids, ox, ovv, ff, m, vir, my_iter);
+  }
+  shake_finish(rows, max_iter_out, vir, my_iter, bid);
+}
+
 __global__ __launch_bounds__(kBlock) void k_shake(SysView s, IntView iv, double* rows, int* max_iter_out) {
   shake_body(s, iv, rows, max_iter_out, (int)blockIdx.x);
+}
+
+// first_half_atom on values already in registers (x: the old position, v:
+// the velocity, scaled in place to the saved old velocity), storing only what
+// SHAKE does not overwrite: the saved velocity, image flags, status, cell.
+// The same operations as first_half_atom (both without FMA contraction, so
+// the compiler cannot round them differently in the two kernels).
+__device__ __forceinline__ int first_half_regs(const SysView& s, const IntView& iv, const double* scale, int do_bin,
+                                               int i, const D3& x, D3& v, const D3& f, double m) {
+#pragma clang fp contract(off)
+  double vx = v.x, vy = v.y, vz = v.z;
+  if (scale) {
+    double fs = *scale;
+    vx = fs * vx;
+    vy = fs * vy;
+    vz = fs * vz;
+  }
+  s.ovx[i] = vx;
+  s.ovy[i] = vy;
+  s.ovz[i] = vz;
+  v = {vx, vy, vz};
+  vx += 0.5 * f.x / m * iv.dt * iv.fmt2v;
+  vy += 0.5 * f.y / m * iv.dt * iv.fmt2v;
+  vz += 0.5 * f.z / m * iv.dt * iv.fmt2v;
+  double p[3] = {x.x, x.y, x.z};
+  p[0] += vx * iv.dt;
+  p[1] += vy * iv.dt;
+  p[2] += vz * iv.dt;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (p[d] < s.lo[d]) {
+      p[d] += s.box[d];
+      s.img[3 * i + d] -= 1;
+    } else if (p[d] >= s.hi[d]) {
+      p[d] -= s.box[d];
+      s.img[3 * i + d] += 1;
+    }
+  }
+  if (!isfinite(p[0] + p[1] + p[2] + vx + vy + vz)) atomicOr(s.status, 1);
+  if (!do_bin) return -1;
+  int ca = bin_axis(p[0], s.lo[0], s.dxdydz[0], s.dims[0]);
+  int cb = bin_axis(p[1], s.lo[1], s.dxdydz[1], s.dims[1]);
+  int cc = bin_axis(p[2], s.lo[2], s.dxdydz[2], s.dims[2]);
+  const int cell = ca + cb * s.dims[0] + cc * s.dims[0] * s.dims[1];
+  s.cell_of[i] = cell;
+  return cell;
+}
+
+// The first half step and SHAKE in one launch, for systems whose every atom
+// is in a SHAKE triplet: the thread of molecule a runs first_half_atom on its
+// three atoms from registers, counts their cells, then runs SHAKE on the same
+// registers.  Bit for bit the two launches: the same arithmetic, with the
+// kicked velocity, drifted position and saved old position that SHAKE
+// replaces (or that only it reads) left unstored; the cells are counted from
+// the drifted positions, as the reference's locator bins before SHAKE.
+__global__ __launch_bounds__(kBlock) void k_first_half_shake(SysView s, IntView iv, const double* scale, int do_bin,
+                                                             double* rows, int* max_iter_out) {
+  extern __shared__ int lds_cells[];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  int cell[3] = {-1, -1, -1};
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  int my_iter = 0;
+  int ids[3];
+  D3 ox[3], ovv[3], ff[3];
+  double m[3];
+  if (a < iv.nmol) {
+    for (int t = 0; t < 3; ++t) {  // every load first, then the arithmetic
+      ids[t] = iv.angle_atoms[3 * a + t];
+      ox[t] = ld(s.x, s.y, s.z, ids[t]);
+      ovv[t] = ld(s.vx, s.vy, s.vz, ids[t]);
+      ff[t] = ld(s.fx, s.fy, s.fz, ids[t]);
+      m[t] = s.mass[ids[t]];
+    }
+    for (int t = 0; t < 3; ++t)
+      cell[t] = first_half_regs(s, iv, scale, do_bin, ids[t], ox[t], ovv[t], ff[t], m[t]);
+  }
+  if (do_bin) count_cells3(s, cell, lds_cells);
+  if (a < iv.nmol) shake_mol(s, iv, ids, ox, ovv, ff, m, vir, my_iter);
+  shake_finish(rows, max_iter_out, vir, my_iter, (int)blockIdx.x);
+}
+
+void launch_first_half_shake(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin,
+                             double* rows, int* max_iter) {
+  const int grid = (iv.nmol + kBlock - 1) / kBlock;
+  const size_t lds = (do_bin && s.lds_bins) ? (size_t)s.ncell * sizeof(int) : 0;
+  if (grid > 0)
+    hipLaunchKernelGGL(k_first_half_shake, dim3(grid), dim3(kBlock), lds, st, s, iv, scale, do_bin, rows, max_iter);
 }
 
 // atoms outside every SHAKE triplet get x = x_old, v = v_old: the reference
@@ -632,9 +744,8 @@ __device__ void compute_pressure(NhcState& S, const NhcParams& p, const double* 
 // functions compile with FMA contraction off, so every kernel they are
 // inlined into (single-system and packed) evaluates the same separately
 // rounded operations and the chain variables agree to the bit.
-__device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, const double* sums, int phase) {
+__device__ __forceinline__ void nhc_phase(NhcState& S, const NhcParams& p, const double* sums, int phase) {
 #pragma clang fp contract(off)
-  NhcState S = *state;
   switch (phase) {
     case 3:  // SetUp: ComputeTempe, ComputeTempTarget, chain masses (ExecutionNVT.cpp:1106-1122)
       compute_tempe(S, p, sums[S_KE]);
@@ -645,6 +756,7 @@ __device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, 
         S.eta_dotdot[c] = (S.eta_mass[c - 1] * S.eta_dot[c - 1] * S.eta_dot[c - 1] - p.boltz * S.t_target) / S.eta_mass[c];
       compute_pressure(S, p, sums);
       S.factor = 1.0;
+      S.factor_end = 1.0;
       break;
     case 1:  // InitialIntegrate: TakeStep increments the step, then ComputeTempTarget + NHC
       S.step += 1;
@@ -663,6 +775,7 @@ __device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, 
         nhc_integrate(S, p);
       else
         S.factor = 1.0;
+      S.factor_end = S.factor;
       break;
     case 4: {  // Berendsen / rescale: ComputeTempe, rescale, pressure (ExecutionNVT.cpp:90-120, 270-291)
       compute_tempe(S, p, sums[S_KE]);
@@ -672,6 +785,7 @@ __device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, 
       else
         coeff = sqrt(1.0 + (p.dt / p.t_period) * (p.t_start / S.tempT - 1.0));
       S.factor = coeff;
+      S.factor_end = coeff;
       compute_pressure(S, p, sums);
       break;
     }
@@ -682,6 +796,21 @@ __device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, 
       break;
     default:
       break;
+  }
+}
+
+// phase 6: FinalIntegrate of this step (2) and then InitialIntegrate of the
+// next (1), in one launch between two steps of a call: phase 1 reads only
+// the state phase 2 leaves (the end-of-step scale stays in factor_end for the
+// velocity scaling that follows).  The same operations in the same order as
+// the two launches.
+__device__ __forceinline__ void nhc_update(NhcState* state, const NhcParams& p, const double* sums, int phase) {
+  NhcState S = *state;
+  if (phase == 6) {
+    nhc_phase(S, p, sums, 2);
+    nhc_phase(S, p, sums, 1);
+  } else {
+    nhc_phase(S, p, sums, phase);
   }
   *state = S;
 }
@@ -781,7 +910,7 @@ __global__ __launch_bounds__(kBlock) void kp_second_half(PACK_PARAMS, int kick, 
 
 __global__ __launch_bounds__(kBlock) void kp_scale_v(PACK_PARAMS) {
   PACK_ENTRY(e, bid)
-  scale_v_body(e.v, &e.nhc->factor, bid);
+  scale_v_body(e.v, &e.nhc->factor_end, bid);
 }
 
 __global__ __launch_bounds__(kBlock) void kp_rattle(PACK_PARAMS) {

@@ -46,6 +46,24 @@ __device__ __forceinline__ int bin_axis(double x, double lo, double d, int dims)
 // first aggregates in LDS, so each cell takes one global atomic per workgroup
 // instead of one per atom.  Every thread of the workgroup must call this
 // (cell < 0: nothing to count).
+// count_cells for three cells per thread (-1: none)
+__device__ __forceinline__ void count_cells3(const SysView& s, const int (&cell)[3], int* lds) {
+  if (s.lds_bins) {
+    for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) lds[c] = 0;
+    __syncthreads();
+    for (int t = 0; t < 3; ++t)
+      if (cell[t] >= 0) atomicAdd(&lds[cell[t]], 1);
+    __syncthreads();
+    for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) {
+      int v = lds[c];
+      if (v) atomicAdd(&s.cell_count[c], v);
+    }
+  } else {
+    for (int t = 0; t < 3; ++t)
+      if (cell[t] >= 0) atomicAdd(&s.cell_count[cell[t]], 1);
+  }
+}
+
 __device__ __forceinline__ void count_cells(const SysView& s, int cell, int* lds) {
   if (s.lds_bins) {
     for (int c = threadIdx.x; c < s.ncell; c += blockDim.x) lds[c] = 0;
