@@ -2099,7 +2099,9 @@ rbmd_status rbmd_pack_create(rbmd_ctx* ctx, rbmd_sys* const* systems, int32_t n,
   rbmd_sys* s0 = systems[0];
   if (!s0) return fail(RBMD_ERR_ARG, "null system 0");
   auto mode_of = [](const rbmd_sys* s) { return s->v.lj_only ? 2 : (s->v.weighted ? 1 : 0); };
-  auto sp_of = [](const rbmd_sys* s) { return s->v.weighted && !s->v.lj_only ? (s->v.sp_max <= 4 ? 4 : 16) : 0; };
+  auto sp_of = [](const rbmd_sys* s) {
+    return s->v.weighted && !s->v.lj_only ? (s->v.sp_max <= 2 ? 2 : (s->v.sp_max <= 4 ? 4 : 16)) : 0;
+  };
   for (int k = 0; k < n; ++k) {
     const rbmd_sys* s = systems[k];
     if (!s) return fail(RBMD_ERR_ARG, "null system %d", k);

@@ -15,8 +15,8 @@ constexpr int kChunk = 64;      // i-atoms per neighbour-build work item (one wa
 constexpr int kMaxSpecial = 8;  // special-list entries held in registers per atom (RBL kernels)
 constexpr int kShiftBits = 9;   // 3 bits per axis image shift in packed neighbour entries
 constexpr int kMaxLanes = 16;   // most lanes sharing one atom's neighbour list
-constexpr int kCoulM = 16;      // Coulomb table intervals over [0, rc]
-constexpr int kCoulD = 11;      // polynomial degree per interval
+constexpr int kCoulM = 32;      // Coulomb table intervals over [0, rc]
+constexpr int kCoulD = 9;       // polynomial degree per interval
 constexpr int kCoulPairs = (kCoulD + 1) / 2;           // coefficient pairs per interval
 constexpr int kCoulTab = 2 * kCoulPairs * kCoulM;      // double2 entries (P and Q)
 constexpr int kCoulD32 = 5;     // degree of the FP32 table of the mixed-precision path

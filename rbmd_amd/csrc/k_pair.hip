@@ -544,6 +544,8 @@ void launch_pair(const SysView& s, hipStream_t st, int thermo, int peratom, doub
     launch_pair_t<2, 0>(s, st, thermo, peratom, rows, grid);
   else if (!s.weighted)
     launch_pair_t<0, 0>(s, st, thermo, peratom, rows, grid);
+  else if (s.sp_max <= 2)  // rigid water: two partners per atom
+    launch_pair_t<1, 2>(s, st, thermo, peratom, rows, grid);
   else if (s.sp_max <= 4)
     launch_pair_t<1, 4>(s, st, thermo, peratom, rows, grid);
   else
@@ -582,6 +584,8 @@ void launch_pack_pair(const PackLaunch& L, int nblocks, const PackSig& g, hipStr
     launch_kp_pair<2, 0>(L, nblocks, g.f32, g.thermo, st);
   else if (g.pair_mode == 0)
     launch_kp_pair<0, 0>(L, nblocks, g.f32, g.thermo, st);
+  else if (g.pair_sp <= 2)
+    launch_kp_pair<1, 2>(L, nblocks, g.f32, g.thermo, st);
   else if (g.pair_sp <= 4)
     launch_kp_pair<1, 4>(L, nblocks, g.f32, g.thermo, st);
   else
