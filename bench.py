@@ -530,10 +530,83 @@ def child_leg(argv, timeout):
     return leg
 
 
+def kernel_label(full):
+    """the bench's label (ctx.profile_records) of a kernel's full name in a rocprofv3 database"""
+    name = full.replace("void ", "").replace("rbmd::", "").split("(")[0]
+    base, _, targs = name.partition("<")
+    targs = [t.strip() for t in targs.rstrip(">").split(",")] if targs else []
+    if base == "k_pair_verlet" and len(targs) > 1:  # <MODE, THERMO, ...>
+        return "pair_thermo" if targs[1] == "true" else "pair"
+    if base == "k_kforce" and targs:  # <THERMO, ...>
+        return "kforce_thermo" if targs[0] == "true" else "kforce"
+    if base == "k_nbr_build_verlet":
+        return "nbr_build"
+    return base[2:] if base.startswith("k_") else base
+
+
+def pmc_means(run_dir, counter):
+    """per-kernel mean over dispatches of one counter (summed over its instances) in a rocprofv3 --pmc run"""
+    import collections
+    import glob
+    import sqlite3
+
+    dbs = glob.glob(os.path.join(run_dir, "**", "*.db"), recursive=True)
+    if len(dbs) != 1:
+        raise SystemExit(f"{run_dir}: expected one rocprofv3 database, found {len(dbs)}")
+    con = sqlite3.connect(dbs[0])
+    per = collections.defaultdict(float)
+    for kname, disp, v in con.execute("select kernel_name, dispatch_id, value from counters_collection "
+                                      "where counter_name = ?", (counter,)):
+        per[(kname, disp)] += v
+    con.close()
+    acc = collections.defaultdict(list)
+    for (kname, _), v in per.items():
+        acc[kname].append(v)
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def traffic_summary(argv):
+    """write profiles/hbm_traffic.json from four rocprofv3 --pmc runs: FETCH_SIZE and WRITE_SIZE of a
+    bench run, then of --copy-calibrate MIB (argv: fetch_dir write_dir copy_fetch_dir copy_write_dir MIB
+    bench_args...)"""
+    hf, hw, cf, cw, mib = argv[:5]
+    bench_args = " ".join(argv[5:])
+    copy_kib = int(mib) * 1024.0
+    cfm, cwm = pmc_means(cf, "FETCH_SIZE"), pmc_means(cw, "WRITE_SIZE")
+    ck = max(cfm, key=lambda k: cfm[k][0])  # the copy kernel: the largest reader of the calibration run
+    scale_f, scale_w = copy_kib / cfm[ck][0], copy_kib / cwm[ck][0]
+    fm, wm = pmc_means(hf, "FETCH_SIZE"), pmc_means(hw, "WRITE_SIZE")
+    kernels = {}
+    for full, (f, n) in sorted(fm.items(), key=lambda kv: -kv[1][0]):
+        if full not in wm or "rbmd::" not in full:
+            continue
+        w = wm[full][0]
+        kernels[kernel_label(full)] = {"kernel": full, "dispatches": n, "fetch_kib": round(f, 1),
+                                       "write_kib": round(w, 1),
+                                       "bytes_per_launch": round((f * scale_f + w * scale_w) * 1024)}
+    out = {"bench_args": bench_args,
+           "commands": [f"rocprofv3 --pmc FETCH_SIZE -d {hf} -- python3 bench.py ... {bench_args}",
+                        f"rocprofv3 --pmc WRITE_SIZE -d {hw} -- python3 bench.py ... {bench_args}",
+                        f"rocprofv3 --pmc FETCH_SIZE -d {cf} -- python3 bench.py --copy-calibrate {mib}",
+                        f"rocprofv3 --pmc WRITE_SIZE -d {cw} -- python3 bench.py --copy-calibrate {mib}",
+                        "python3 bench.py --traffic-summary " + " ".join(argv)],
+           "note": ("per-dispatch means; bytes_per_launch = fetch_kib*scale_fetch*1024 + write_kib*scale_write*1024;"
+                    f" the scales make the counters read {mib} MiB for a {mib} MiB device copy ({ck}); the _thermo"
+                    " entries are the last step of each call (virial and energy sums)"),
+           "scale_fetch": round(scale_f, 4), "scale_write": round(scale_w, 4), "kernels": kernels}
+    with open(os.path.join(ROOT, "profiles", "hbm_traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps({k: v["bytes_per_launch"] for k, v in kernels.items()}))
+
+
 def traffic_key(workload, precision, skin):
     """the bench arguments an HBM-traffic record was measured with"""
     return f"--workload {workload} --precision {precision} --skin {skin:g}"
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "--traffic-summary":
+        traffic_summary(sys.argv[2:])
+    else:
+        main()
