@@ -418,15 +418,23 @@ __device__ void misfit_block(const SysView& s, int wave, int lane, int slot) {
   }
 }
 
-// One workgroup: kNbrWaves columns of one work item (or of one misfit slot);
-// block bid is part bid % kNbrParts of work item bid / kNbrParts.
+// One workgroup: kNbrWaves columns of one work item (or of one misfit slot).
+// The grid is kNbrParts passes over the work items (and misfit slots); pass 0
+// holds part 1 of every item -- the centre column and two edge columns, whose
+// box culls keep the most candidates (about 2.8, 2.2 and 2.2 cells' worth
+// against 1.4 for a corner column) -- so the longest-lived wavefronts are
+// dispatched first; passes 1 and 2 hold parts 2 and 0.
+static_assert(kNbrParts == 3, "the pass order below names three parts");
 __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const int bid) {
   __shared__ float4 stage[kNbrWaves][kStageCap];  // FP32 position from the chunk's box corner, entry bits in w
   __shared__ double4 stage64[kNbrWaves][kStageCap];  // the same candidates' shifted FP64 positions (re-tests only)
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
-  const int wid = bid / kNbrParts;
-  const int wave = (bid % kNbrParts) * kNbrWaves + w;  // the stencil column this wavefront owns
+  const int per_pass = s.n_items + kMisfitBlocks;
+  const int pass = bid / per_pass;
+  const int wid = bid - pass * per_pass;
+  const int part = pass == kNbrParts - 1 ? 0 : pass + 1;
+  const int wave = part * kNbrWaves + w;  // the stencil column this wavefront owns
   if (!list_rebuild(s)) return;  // skin list between builds
   if (wid >= s.n_items) {
     misfit_block(s, wave, lane, wid - s.n_items);
