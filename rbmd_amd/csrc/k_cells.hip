@@ -488,38 +488,46 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   // the list cutoff of the box, so each coordinate is below 3 rc in size and
   // carries at most 2^-24 * 3 rc of conversion error; with the FP32 difference
   // and sum of squares, |d2_f32 - d2_f64| < 1.7e-6 rc^2 where d2 is near rc^2.
-  // Candidates within 1e-5 rc^2 of rc^2 - 1e-4, or below 1e-3, are decided by
-  // the FP64 predicate of the per-candidate path (the reference's
+  // Candidates within 1e-5 rc^2 of rc^2 - 1e-4, or at or below 1e-3, are
+  // decided by the FP64 predicate of the per-candidate path (the reference's
   // rc^2 - d^2 > 1e-4 && d^2 > 1e-4 on the same rounded vector), all others by
   // the FP32 comparison, which then agrees with it.  The atom itself (same
-  // cell, no image shift) is rejected by its entry: its d^2 is exactly 0.
+  // cell, no image shift) has d^2 = 0 exactly, so the FP64 predicate rejects
+  // it.  A lane without an i-atom sits at 1e30: its d^2 is +inf, and neither
+  // predicate holds.
   const float cut2m_f = (float)(cut2 - 0.0001);
   const float band_f = (float)(1e-5 * cut2);
-  const float lo_f = 1e-3f;
-  const float lix = (float)(pi.x - bl[0]), liy = (float)(pi.y - bl[1]), liz = (float)(pi.z - bl[2]);
-  const int self_entry = active ? pack_entry(i, 0, 0, 0) : -1;
-  // Accepted entries collect in four registers and leave as one 16-byte store
-  // per four entries (a segment starts on a 64-byte boundary and holds a
-  // multiple of 16 slots), so the row is written in whole 16-byte pieces
-  // rather than one scattered 4-byte store per entry.
+  // accepted in FP32: 1e-3 < d2 < cut2m - band, as one unsigned range test on
+  // the bits (non-negative floats order as their bit patterns)
+  const unsigned in_lo = __float_as_uint(1e-3f) + 1u;
+  const unsigned in_w = __float_as_uint(cut2m_f - band_f) - in_lo;
+  const float out_f = cut2m_f + band_f;  // rejected in FP32: d2 >= out_f
+  const float lix = active ? (float)(pi.x - bl[0]) : 1e30f;
+  const float liy = active ? (float)(pi.y - bl[1]) : 1e30f;
+  const float liz = active ? (float)(pi.z - bl[2]) : 1e30f;
+  // Accepted entries shift through four registers (the newest in b3) and
+  // leave as one 16-byte store per four entries (a segment starts on a
+  // 64-byte boundary and holds a multiple of 16 slots), so the row is written
+  // in whole 16-byte pieces rather than one scattered 4-byte store per entry.
+  // The shift is four selects on the accept mask, with no branch around it.
   int cnt = 0;
   int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-  auto push = [&](int e) {
-    const int k = cnt & 3;
-    b0 = k == 0 ? e : b0;
-    b1 = k == 1 ? e : b1;
-    b2 = k == 2 ? e : b2;
-    b3 = k == 3 ? e : b3;
-    if (k == 3 && cnt < seg_cap) *reinterpret_cast<int4*>(row + (cnt - 3)) = make_int4(b0, b1, b2, b3);
-    ++cnt;
+  auto push = [&](bool ok, int e) {
+    b0 = ok ? b1 : b0;
+    b1 = ok ? b2 : b1;
+    b2 = ok ? b3 : b2;
+    b3 = ok ? e : b3;
+    cnt += ok ? 1 : 0;
+    // past the capacity the last four slots are overwritten: the list is then
+    // flagged as overflowed (status 4) and not used
+    if (ok & ((cnt & 3) == 0)) *reinterpret_cast<int4*>(row + (min(cnt, seg_cap) - 4)) = make_int4(b0, b1, b2, b3);
   };
   // the FP32 predicate of one staged candidate; nr: decide it in FP64
   auto test = [&](const float4 q, bool& ok, bool& nr) {
     const float rx = q.x - lix, ry = q.y - liy, rz = q.z - liz;
     const float d2 = rx * rx + ry * ry + rz * rz;
-    const bool live = active & (__float_as_int(q.w) != self_entry);
-    ok = live & (d2 < cut2m_f) & (d2 > lo_f);
-    nr = live & ((fabsf(d2 - cut2m_f) < band_f) | (d2 <= lo_f));
+    ok = __float_as_uint(d2) - in_lo < in_w;
+    nr = !ok & (d2 < out_f);
   };
   // the per-candidate FP64 predicate on the build's own vector (the staged
   // shifted position; no global load in the test loop, where a load would
@@ -583,30 +591,33 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
       bool ok[4], nr[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) test(q[r], ok[r], nr[r]);
-      if (__ballot(nr[0] | nr[1] | nr[2] | nr[3]) != 0ull) {
+      // two copies of the appends, so that in the common one the FP32
+      // decisions stay lane masks
+      if (__ballot(nr[0] | nr[1] | nr[2] | nr[3]) == 0ull) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (nr[r]) ok[r] = exact(u + r);
+        for (int r = 0; r < 4; ++r) push(ok[r], __float_as_int(q[r].w));
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) push(nr[r] ? exact(u + r) : ok[r], __float_as_int(q[r].w));
       }
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (ok[r]) push(__float_as_int(q[r].w));
     }
     for (; u < kc; ++u) {
       const float4 q = stage[w][u];
       bool ok, nr;
       test(q, ok, nr);
       if (nr) ok = exact(u);
-      if (ok) push(__float_as_int(q.w));
+      push(ok, __float_as_int(q.w));
     }
     wave_sync();
   }
-  {  // the last (cnt & 3) entries
+  {  // the last (cnt & 3) entries: b(4 - k) ... b3
     const int k = cnt & 3;
     const int first = cnt - k;
-    if (k > 0 && first < seg_cap) row[first] = b0;
-    if (k > 1 && first + 1 < seg_cap) row[first + 1] = b1;
-    if (k > 2 && first + 2 < seg_cap) row[first + 2] = b2;
+    const int e0 = k == 3 ? b1 : (k == 2 ? b2 : b3);
+    const int e1 = k == 3 ? b2 : b3;
+    if (k > 0 && first < seg_cap) row[first] = e0;
+    if (k > 1 && first + 1 < seg_cap) row[first + 1] = e1;
+    if (k > 2 && first + 2 < seg_cap) row[first + 2] = b3;
   }
   if (cnt > seg_cap) atomicOr(s.status, 4);
   if (active) s.nbr_num[wave * s.n + i] = cnt < seg_cap ? cnt : seg_cap;
