@@ -103,6 +103,18 @@ __device__ __forceinline__ double coul_poly(const double2* ctab, int k, double u
 // per kernel (by what else reads a product), and the kernels with and without
 // the energy and virial sums, alone and packed, would round the same force
 // differently.
+// 1/sqrt(x) for a positive finite x (a pair's d2 > 1e-4): the hardware
+// estimate y refined by the second-order series of (1 - e)^(-1/2) in the
+// residual e = 1 - x y^2, y (1 + e/2 + 3 e^2 / 8).  It is the device
+// library's rsqrt without that function's select for zero and infinite
+// arguments, which cannot occur here, so the value is the same to the bit.
+__device__ __forceinline__ double rsqrt_pos(double x) {
+#pragma clang fp contract(off)
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(y * -x, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 template <int CHECK>
 __device__ __forceinline__ bool pair_in(const SysView& s, double d2) {
   if (CHECK == 0) return true;
@@ -119,7 +131,7 @@ __device__ __forceinline__ double pair_term(const SysView& s, const double2* cta
   elj = 0;
   ec = 0;
   if (in) {
-    const double rinv = rsqrt(d2);
+    const double rinv = rsqrt_pos(d2);
     const double rinv2 = rinv * rinv;
     const double sd6 = sig6 * (rinv2 * rinv2 * rinv2);
     flj = -24.0 * epsij * (fma(2.0, sd6, -1.0) * sd6) * rinv2;
