@@ -176,6 +176,7 @@ struct rbmd_sys {
   // captured MD steps (rbmd_md_run_graph): [0] without, [1] with the thermo sums
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graph_ok = false;           // cleared whenever captured pointers or sizes change
+  bool rattle_pending = false;     // within an eager call: the last step's RATTLE runs with the next step's start
   double* kring = nullptr; // [2R][3P]
   double* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -1743,8 +1744,12 @@ static void enqueue_step_start(rbmd_sys* s) {
 
 // One MD step.  start: the step begins with its own thermostat launch (else
 // the previous step's end ran it); merge_end: the end-of-step thermostat
-// launch also starts the next step (every step of a call but the last).
-static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool thermo, bool start, bool merge_end) {
+// launch also starts the next step (every step of a call but the last);
+// defer_rattle: when every atom is constrained, the step's RATTLE is left to
+// the next step's first launch (k_rattle_first_half_shake), which then must
+// follow in the same call (rattle_pending; flush_rattle launches it alone).
+static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool thermo, bool start, bool merge_end,
+                         bool defer_rattle = false) {
   rbmd_ctx* c = s->ctx;
   hipStream_t st = c->stream;
   const int tc = s->p.temp_ctrl;
@@ -1757,9 +1762,16 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool the
   // also bins (per-step lists: the reference's locator bins before SHAKE)
   const int do_bin = s->skin_active ? 0 : 1;
   if (shake && s->nfree == 0) {  // every atom constrained: one launch for both
-    c->run("first_half_shake", [&] {
-      launch_first_half_shake(s->v, s->iv, st, scale, do_bin, s->seg_rows(SEG_SHAKE), s->shake_iter);
-    });
+    if (s->rattle_pending) {  // and the previous step's RATTLE in front of them
+      c->run("rattle_first_half_shake", [&] {
+        launch_rattle_first_half_shake(s->v, s->iv, st, scale, do_bin, s->seg_rows(SEG_SHAKE), s->shake_iter);
+      });
+      s->rattle_pending = false;
+    } else {
+      c->run("first_half_shake", [&] {
+        launch_first_half_shake(s->v, s->iv, st, scale, do_bin, s->seg_rows(SEG_SHAKE), s->shake_iter);
+      });
+    }
     s->seg_used[SEG_SHAKE] = (s->nmol + kBlock - 1) / kBlock;
   } else {
     c->run("first_half", [&] { launch_first_half(s->v, s->iv, st, scale, do_bin); });
@@ -1802,7 +1814,9 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool the
     });
     if (tc == RBMD_TEMP_NOSE_HOOVER && !(shake && s->nfree == 0))
       c->run("scale_v", [&] { launch_scale_v(s->v, st, &s->nhc->factor_end); });
-    if (shake) {
+    if (shake && s->nfree == 0 && defer_rattle && merge_end) {
+      s->rattle_pending = true;
+    } else if (shake) {
       c->run("rattle", [&] { launch_rattle(s->v, s->iv, st); });
       if (s->nfree) launch_unconstrained_copy(s->v, st, s->free_dev, s->nfree, 0);
     }
@@ -1817,11 +1831,18 @@ static void enqueue_step(rbmd_sys* s, const double* kv, bool from_ring, bool the
   }
 }
 
+// the RATTLE a step left to the next one, alone (a call that stops early)
+static void flush_rattle(rbmd_sys* s) {
+  if (!s->rattle_pending) return;
+  s->ctx->run("rattle", [&] { launch_rattle(s->v, s->iv, s->ctx->stream); });
+  s->rattle_pending = false;
+}
+
 static rbmd_status md_one_step(rbmd_sys* s, bool first, bool last) {
   const int64_t step = s->step + 1;
   const double* kv = nullptr;
   RC_TRY(kvec_for_step(s, step, &kv));
-  enqueue_step(s, kv, false, last, first, nhc_merges(s) && !last);
+  enqueue_step(s, kv, false, last, first, nhc_merges(s) && !last, !last);
   s->step = step;
   return RBMD_OK;
 }
@@ -1864,12 +1885,17 @@ rbmd_status rbmd_md_step(rbmd_sys* s, int64_t n_steps) {
   if (!s->initialized) return fail(RBMD_ERR_STATE, "rbmd_md_init has not run");
   RC_TRY(prepare_skin(s));
   for (int64_t k = 0; k < n_steps; ++k) {
-    RC_TRY(md_one_step(s, k == 0, k == n_steps - 1));
-    if (s->p.energy_every > 0 && has_k(s) && s->step % s->p.energy_every == 0) {
-      RC_TRY(long_energy(s, s->p.kmax, &s->e_long, nullptr));
-      s->energy_step = s->step;
+    rbmd_status rc = md_one_step(s, k == 0, k == n_steps - 1);
+    if (rc == RBMD_OK && s->p.energy_every > 0 && has_k(s) && s->step % s->p.energy_every == 0) {
+      rc = long_energy(s, s->p.kmax, &s->e_long, nullptr);  // positions only: a pending RATTLE may wait
+      if (rc == RBMD_OK) s->energy_step = s->step;
+    }
+    if (rc != RBMD_OK) {
+      flush_rattle(s);
+      return rc;
     }
   }
+  flush_rattle(s);  // a no-op: the last step of a call runs its own RATTLE
   RC_TRY(check_launch("md_step"));
   int stt = 0;
   RC_TRY(read_status(s, &stt));

@@ -268,6 +268,10 @@ void launch_shake(const SysView& s, const IntView& iv, hipStream_t st, double* p
 void launch_first_half_shake(const SysView& s, const IntView& iv, hipStream_t st, const double* scale, int do_bin,
                              double* partials, int* max_iter);
 void launch_rattle(const SysView& s, const IntView& iv, hipStream_t st);
+// RATTLE of one step and the first half step + SHAKE of the next, one launch
+// (systems whose every atom is constrained)
+void launch_rattle_first_half_shake(const SysView& s, const IntView& iv, hipStream_t st, const double* scale,
+                                    int do_bin, double* rows, int* max_iter);
 void launch_bonded(const SysView& s, const IntView& iv, hipStream_t st, int thermo, double* partials, int* nblocks);
 void launch_second_half(const SysView& s, const IntView& iv, hipStream_t st, int kick, int has_k, int has_b,
                         double* partials, int* nblocks);

@@ -88,12 +88,32 @@ void launch_first_half(const SysView& s, const IntView& iv, hipStream_t st, cons
 struct D3 {
   double x, y, z;
 };
-__device__ __forceinline__ D3 sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ D3 add(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ D3 mul(double k, D3 a) { return {k * a.x, k * a.y, k * a.z}; }
-__device__ __forceinline__ D3 divs(D3 a, double k) { return {a.x / k, a.y / k, a.z / k}; }
-__device__ __forceinline__ double dotp(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// The vector helpers, SHAKE and RATTLE compile with FMA contraction off: a
+// product and a sum written in separate helpers are then rounded separately
+// in every kernel that inlines them, so the fused and the separate launches
+// of these steps agree to the bit.
+__device__ __forceinline__ D3 sub(D3 a, D3 b) {
+#pragma clang fp contract(off)
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+__device__ __forceinline__ D3 add(D3 a, D3 b) {
+#pragma clang fp contract(off)
+  return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+__device__ __forceinline__ D3 mul(double k, D3 a) {
+#pragma clang fp contract(off)
+  return {k * a.x, k * a.y, k * a.z};
+}
+__device__ __forceinline__ D3 divs(D3 a, double k) {
+#pragma clang fp contract(off)
+  return {a.x / k, a.y / k, a.z / k};
+}
+__device__ __forceinline__ double dotp(D3 a, D3 b) {
+#pragma clang fp contract(off)
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
 __device__ __forceinline__ D3 mimg(const SysView& s, D3 a, D3 b) {
+#pragma clang fp contract(off)
   D3 v = sub(a, b);
   if (fabs(v.x) > s.box[0] * 0.5) v.x -= (v.x > 0 ? s.box[0] : -s.box[0]);
   if (fabs(v.y) > s.box[1] * 0.5) v.y -= (v.y > 0 ? s.box[1] : -s.box[1]);
@@ -113,6 +133,7 @@ __device__ __forceinline__ void st3(double* x, double* y, double* z, int i, D3 v
 __device__ __forceinline__ void shake_mol(const SysView& s, const IntView& iv, const int (&ids)[3], const D3 (&ox)[3],
                                           const D3 (&ovv)[3], const D3 (&ff)[3], const double (&m)[3], double (&vir)[6],
                                           int& my_iter) {
+#pragma clang fp contract(off)
   {
     const double dt = iv.dt, fmt2v = iv.fmt2v;
     D3 sp[3];
@@ -372,21 +393,17 @@ void launch_unconstrained_copy(const SysView& s, hipStream_t st, const int* free
                      nfree, pos);
 }
 
-// RATTLE on (x, old v, F)
-__device__ __forceinline__ void rattle_body(const SysView& s, const IntView& iv, const int bid) {
-  int a = bid * blockDim.x + threadIdx.x;
-  if (a >= iv.nmol) return;
+// RATTLE of one H-O-H triplet on values in registers: the velocity starts
+// again from the saved old velocity ov plus the half kick of the new force f,
+// as the reference's RATTLE does (MolecularWorklet.h:2647-2832); x are the
+// positions; the corrected velocities go to vout.
+__device__ __forceinline__ void rattle_mol(const SysView& s, const IntView& iv, const D3 (&ov)[3], const D3 (&f)[3],
+                                           const double (&m)[3], const D3 (&x)[3], D3 (&vout)[3]) {
+#pragma clang fp contract(off)
   const double dt = iv.dt, fmt2v = iv.fmt2v;
-  int i0 = iv.angle_atoms[3 * a], i1 = iv.angle_atoms[3 * a + 1], i2 = iv.angle_atoms[3 * a + 2];
-  int ids[3] = {i0, i1, i2};
   D3 sv[3];
-  double m[3];
-  for (int t = 0; t < 3; ++t) {
-    int id = ids[t];
-    m[t] = s.mass[id];
-    sv[t] = add(ld(s.ovx, s.ovy, s.ovz, id), mul(fmt2v, divs(mul(0.5 * dt, ld(s.fx, s.fy, s.fz, id)), m[t])));
-  }
-  D3 p0 = ld(s.x, s.y, s.z, i0), p1 = ld(s.x, s.y, s.z, i1), p2 = ld(s.x, s.y, s.z, i2);
+  for (int t = 0; t < 3; ++t) sv[t] = add(ov[t], mul(fmt2v, divs(mul(0.5 * dt, f[t]), m[t])));
+  const D3 p0 = x[0], p1 = x[1], p2 = x[2];
   D3 r01 = mimg(s, p1, p0), r12 = mimg(s, p2, p1), r20 = mimg(s, p0, p2);
   D3 sv10 = sub(sv[0], sv[1]), sv21 = sub(sv[1], sv[2]), sv02 = sub(sv[2], sv[0]);
   double im0 = 1.0 / m[0], im1 = 1.0 / m[1], im2 = 1.0 / m[2];
@@ -423,9 +440,27 @@ __device__ __forceinline__ void rattle_body(const SysView& s, const IntView& iv,
   D3 vc0 = sub(mul(im0, mul(l[0], r01)), mul(im0, mul(l[2], r20)));
   D3 vc1 = sub(mul(im1, mul(l[1], r12)), mul(im1, mul(l[0], r01)));
   D3 vc2 = sub(mul(im2, mul(l[2], r20)), mul(im2, mul(l[1], r12)));
-  st3(s.vx, s.vy, s.vz, i0, add(sv[0], vc0));
-  st3(s.vx, s.vy, s.vz, i1, add(sv[1], vc1));
-  st3(s.vx, s.vy, s.vz, i2, add(sv[2], vc2));
+  vout[0] = add(sv[0], vc0);
+  vout[1] = add(sv[1], vc1);
+  vout[2] = add(sv[2], vc2);
+}
+
+// RATTLE on (x, old v, F) of one H-O-H triplet per thread
+__device__ __forceinline__ void rattle_body(const SysView& s, const IntView& iv, const int bid) {
+  const int a = bid * blockDim.x + threadIdx.x;
+  if (a >= iv.nmol) return;
+  int ids[3];
+  D3 ov[3], f[3], x[3], v[3];
+  double m[3];
+  for (int t = 0; t < 3; ++t) {
+    ids[t] = iv.angle_atoms[3 * a + t];
+    ov[t] = ld(s.ovx, s.ovy, s.ovz, ids[t]);
+    f[t] = ld(s.fx, s.fy, s.fz, ids[t]);
+    x[t] = ld(s.x, s.y, s.z, ids[t]);
+    m[t] = s.mass[ids[t]];
+  }
+  rattle_mol(s, iv, ov, f, m, x, v);
+  for (int t = 0; t < 3; ++t) st3(s.vx, s.vy, s.vz, ids[t], v[t]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_rattle(SysView s, IntView iv) {
@@ -435,6 +470,50 @@ __global__ __launch_bounds__(kBlock) void k_rattle(SysView s, IntView iv) {
 void launch_rattle(const SysView& s, const IntView& iv, hipStream_t st) {
   int grid = (iv.nmol + kBlock - 1) / kBlock;
   if (grid > 0) hipLaunchKernelGGL(k_rattle, dim3(grid), dim3(kBlock), 0, st, s, iv);
+}
+
+// The end of one step and the start of the next in one launch, for systems
+// whose every atom is in a SHAKE triplet: the thread of molecule a runs RATTLE
+// on its three atoms (the step's end), then, from the same registers, the
+// next step's first half step, the cell count and SHAKE, as
+// k_first_half_shake does.  Nothing between the two reads a velocity: the
+// thermostat's step boundary has already run (it reads the kinetic energy
+// before RATTLE), and RATTLE leaves the positions and forces as they are.
+// Bit for bit the two launches k_rattle and k_first_half_shake.
+__global__ __launch_bounds__(kBlock) void k_rattle_first_half_shake(SysView s, IntView iv, const double* scale,
+                                                                    int do_bin, double* rows, int* max_iter_out) {
+  extern __shared__ int lds_cells[];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  int cell[3] = {-1, -1, -1};
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  int my_iter = 0;
+  int ids[3];
+  D3 x[3], v[3], f[3];
+  double m[3];
+  if (a < iv.nmol) {
+    D3 ov[3];
+    for (int t = 0; t < 3; ++t) {  // every load first, then the arithmetic
+      ids[t] = iv.angle_atoms[3 * a + t];
+      ov[t] = ld(s.ovx, s.ovy, s.ovz, ids[t]);
+      f[t] = ld(s.fx, s.fy, s.fz, ids[t]);
+      x[t] = ld(s.x, s.y, s.z, ids[t]);
+      m[t] = s.mass[ids[t]];
+    }
+    rattle_mol(s, iv, ov, f, m, x, v);
+    for (int t = 0; t < 3; ++t) cell[t] = first_half_regs(s, iv, scale, do_bin, ids[t], x[t], v[t], f[t], m[t]);
+  }
+  if (do_bin) count_cells3(s, cell, lds_cells);
+  if (a < iv.nmol) shake_mol(s, iv, ids, x, v, f, m, vir, my_iter);
+  shake_finish(rows, max_iter_out, vir, my_iter, (int)blockIdx.x);
+}
+
+void launch_rattle_first_half_shake(const SysView& s, const IntView& iv, hipStream_t st, const double* scale,
+                                    int do_bin, double* rows, int* max_iter) {
+  const int grid = (iv.nmol + kBlock - 1) / kBlock;
+  const size_t lds = (do_bin && s.lds_bins) ? (size_t)s.ncell * sizeof(int) : 0;
+  if (grid > 0)
+    hipLaunchKernelGGL(k_rattle_first_half_shake, dim3(grid), dim3(kBlock), lds, st, s, iv, scale, do_bin, rows,
+                       max_iter);
 }
 
 // bond + angle forces gathered per atom over its incident terms; the energy
