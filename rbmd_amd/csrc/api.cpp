@@ -1024,6 +1024,10 @@ rbmd_status rbmd_sys_create(rbmd_ctx* ctx, const rbmd_sys_desc* d, const rbmd_pa
   iv.nb = s->nb;
   iv.na = s->na;
   iv.shake_tol = p->shake_tol > 0 ? p->shake_tol : 0.00001;
+  {  // the rigid geometry's H-H distance, once, as the oracle forms it
+    const double b1 = 1.0, b2 = 1.0;
+    iv.shake_d12 = std::sqrt(b1 * b1 + b2 * b2 - 2.0 * b1 * b2 * std::cos((109.4700 / 180.0) * kPi));
+  }
   // molecule groups of SpecialCoulGeneral: every atom lists the atoms of its
   // molecule in ascending id (SetMolecularGroup, ModelFileInitCondition.cpp:698-736);
   // the correction runs for CVFF over random-batch lists with special bonds

@@ -256,6 +256,7 @@ struct IntView {
   const int *inc_b_off, *inc_b;   // atom -> bonds
   const int *inc_a_off, *inc_a;   // atom -> angles
   double shake_tol;               // SHAKE convergence on the multiplier change
+  double shake_d12;               // H-H distance of the rigid geometry (1 A bonds at 109.47 deg), formed on the host
   // SpecialCoulGeneral (CVFF + RBL + special bonds): atom -> the atoms of its
   // molecule, itself included, in ascending id (CSR over atoms)
   int spec_coul;

@@ -140,7 +140,7 @@ __device__ __forceinline__ void shake_mol(const SysView& s, const IntView& iv, c
     for (int t = 0; t < 3; ++t)
       sp[t] = add(add(ox[t], mul(dt, ovv[t])), mul(fmt2v, divs(mul(0.5 * dt * dt, ff[t]), m[t])));
     const double bond1 = 1.0, bond2 = 1.0;
-    const double bond12 = sqrt(bond1 * bond1 + bond2 * bond2 - 2.0 * bond1 * bond2 * cos((109.4700 / 180.0) * kPi));
+    const double bond12 = iv.shake_d12;  // sqrt(b1^2 + b2^2 - 2 b1 b2 cos(109.47 deg)), MolecularWorklet.h's geometry
     D3 r01 = mimg(s, ox[1], ox[0]);
     D3 r12 = mimg(s, ox[2], ox[1]);
     D3 r20 = mimg(s, ox[0], ox[2]);
@@ -248,7 +248,11 @@ __device__ __forceinline__ void shake_mol(const SysView& s, const IntView& iv, c
 // the SHAKE rows of a block and the iteration maximum
 __device__ __forceinline__ void shake_finish(double* rows, int* max_iter_out, const double (&vir)[6], int my_iter,
                                              const int bid) {
-  if (max_iter_out) atomicMax(max_iter_out, my_iter);
+  if (max_iter_out) {  // the wavefront's maximum, one atomic per wavefront (not one per molecule)
+    int mx = my_iter;
+    for (int off = 1; off < kWave; off <<= 1) mx = max(mx, __shfl_xor(mx, off, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(max_iter_out, mx);
+  }
   const double vals[kRowShake] = {vir[0], vir[1], vir[2], vir[3], vir[4], vir[5]};
   block_write_row<kRowShake>(vals, rows + (size_t)bid * kRowShake);
 }
