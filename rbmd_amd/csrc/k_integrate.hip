@@ -740,6 +740,11 @@ void launch_scale_v(const SysView& s, hipStream_t st, const double* scale) {
 // ------------------------------------------------------------------ NHC
 
 
+// e^x, with e^(+-0) = 1 exactly without the call (the last chain element's
+// velocity eta_dot[3] stays 0, so two of the six factors of a half step are
+// exactly 1)
+__device__ __forceinline__ double exp_or_one(double x) { return x == 0.0 ? 1.0 : exp(x); }
+
 __device__ void nhc_integrate(NhcState& S, const NhcParams& p) {
 #pragma clang fp contract(off)
   double T = S.tempT;
@@ -750,7 +755,7 @@ __device__ void nhc_integrate(NhcState& S, const NhcParams& p) {
   const double ncfac = 1.0;
   double expfac;
   for (int c = 2; c > 0; --c) {
-    expfac = exp(-ncfac * p.dt8 * S.eta_dot[c + 1]);
+    expfac = exp_or_one(-ncfac * p.dt8 * S.eta_dot[c + 1]);
     S.eta_dot[c] *= expfac;
     S.eta_dot[c] += S.eta_dotdot[c] * ncfac * p.dt4;
     S.eta_dot[c] *= p.tdrag;
@@ -770,7 +775,7 @@ __device__ void nhc_integrate(NhcState& S, const NhcParams& p) {
   S.eta_dot[0] += S.eta_dotdot[0] * ncfac * p.dt4;
   S.eta_dot[0] *= expfac;
   for (int c = 1; c < 3; ++c) {
-    expfac = exp(-ncfac * p.dt8 * S.eta_dot[c + 1]);
+    expfac = exp_or_one(-ncfac * p.dt8 * S.eta_dot[c + 1]);
     S.eta_dot[c] *= expfac;
     S.eta_dotdot[c] = (S.eta_mass[c - 1] * S.eta_dot[c - 1] * S.eta_dot[c - 1] - p.boltz * S.t_target) / S.eta_mass[c];
     S.eta_dot[c] += S.eta_dotdot[c] * ncfac * p.dt4;
