@@ -361,17 +361,13 @@ void launch_cell_sort(const SysView& s, hipStream_t st) {
 // workgroup per item, a 30,000-atom launch put three items on some CUs and
 // two on others, and the fullest CUs set the kernel time.
 //
-// The three cells of a column form one candidate stream.  Each round, every
-// lane holds one candidate (coalesced load, image shift applied; the next
-// round's load is issued before this round is processed).  Candidates farther
-// than the cutoff from the i-chunk's bounding box are dropped by a
-// wave-uniform ballot and the rest are compacted into LDS (position and entry
-// in one 32-byte slot).  Every lane then tests the compacted candidates
-// against its own i-atom with the reference's FP64 predicate, four candidates
-// per LDS round trip, and appends each accepted entry to its own row at once,
-// so the row stays in stream order.  The kernel is latency-bound at 30,000
-// atoms (one generation of waves, ~30 us each): the four-wide reads cut the
-// exposed LDS waits per candidate, which moved it from 112 to 102 us there.
+// The three cells of a column form one candidate stream.  A batch of up to
+// 128 candidates is loaded at once (image shift applied); candidates farther
+// than the cutoff from the i-chunk's bounding box are dropped by a ballot and
+// the rest staged in LDS in stream order.  Every lane then tests the staged
+// candidates against its own i-atom (FP32, with the reference's FP64
+// predicate near the thresholds) and appends each accepted entry to its own
+// row, so the row stays in stream order.
 //
 // Workgroups past the work items take the misfit atoms (whose current cell is
 // not the binned one, e.g. after SHAKE): one workgroup per misfit atom, the 9
