@@ -63,6 +63,65 @@ def test_neighbor_lists_bit_exact(ctx, kind):
     assert np.array_equal(go, oo)
 
 
+def _pairs_at_the_cutoff(s, cut):
+    """A copy of salt system s with pairs placed at d^2 around the list threshold rc^2 - 1e-4
+    (within and just outside 1e-5 rc^2 of it, and to a few ulps), pairs across the periodic
+    boundary, and pairs near the d^2 > 1e-4 floor; returns (system, pair index list)."""
+    rng = np.random.default_rng(5)
+    x = s.x.copy()
+    L = np.asarray(s.hi) - np.asarray(s.lo)
+    rc2 = cut * cut
+    thr = rc2 - 1e-4
+    pairs = []
+
+    def place(a, b, d2, base=None):
+        u = rng.normal(size=3)
+        if base is not None:
+            x[a] = base
+            u = -np.abs(u)  # the partner lies beyond the lower faces: wrapped to the upper ones
+        u /= np.linalg.norm(u)
+        x[b] = x[a] + u * np.sqrt(d2)
+        pairs.append((a, b))
+
+    k = 0
+    for dlt in list(np.linspace(-2e-5, 2e-5, 41)) + [0.0, 1e-15, -1e-15, 4e-15, -4e-15]:
+        place(2 * k, 2 * k + 1, thr + dlt * rc2)
+        k += 1
+    for d2 in (0.5e-4, 1e-4, 1.5e-4, 9e-4, 1e-3, 1.1e-3):
+        place(2 * k, 2 * k + 1, d2)
+        k += 1
+    for dlt in np.linspace(-1e-5, 1e-5, 11):
+        place(2 * k, 2 * k + 1, thr + dlt * rc2, base=np.asarray(s.lo) + 0.3 * rng.random(3))
+        k += 1
+    x = np.asarray(s.lo) + np.mod(x - np.asarray(s.lo), L)
+    return type(s)(**{**s.__dict__, "x": x}), pairs
+
+
+def test_neighbor_lists_at_the_cutoff(ctx):
+    """The list build decides most candidates in FP32 and re-tests those within 1e-5 rc^2 of the
+    threshold (and below d^2 = 1e-3) with the reference's FP64 predicate: pairs placed in and
+    around that band, to the last bits of the threshold and across the box, give lists equal to
+    the oracle's bit for bit."""
+    r = salt_run()
+    s, pairs = _pairs_at_the_cutoff(salt(n=4000, seed=17), r.cut_off)
+    # the placed pairs do land in the band, on both sides of the threshold (the reference's
+    # rounding of the minimum-image vector; every box edge is above 2 rc here)
+    L = np.asarray(s.hi) - np.asarray(s.lo)
+    thr = r.cut_off ** 2 - 1e-4
+    d = np.array([s.x[b] - s.x[a] for a, b in pairs])
+    d = d - L * np.rint(d / L)
+    d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    band = np.abs(d2 - thr) < 1e-5 * r.cut_off ** 2
+    assert band.sum() >= 30 and (d2[band] < thr).sum() >= 10 and (d2[band] > thr).sum() >= 10
+    assert np.abs(d2 - thr).min() < 1e-14  # some to the last few bits of the threshold
+    g, o = pair(ctx, s, r)
+    gi, go, gn = g.neighbors(400)
+    oi, oo, on = o.neighbors(400)
+    assert np.array_equal(gn, on)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(go, oo)
+
+
 def test_neighbor_counts_bruteforce(ctx):
     s = salt(n=1000, seed=7)
     g, o = pair(ctx, s, salt_run())
