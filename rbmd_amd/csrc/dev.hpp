@@ -214,7 +214,8 @@ int force_cells_blocks(const SysView& s);
 // ---------------------------------------------------------------- k space
 // One k vector as the force pass reads it (written by the rho reduction):
 // k, the prefactor 1/(V k^2) (Ewald: times e^{-k^2/4 alpha}), m = k L / 2 pi
-// and rho(k).  80 bytes, read with scalar loads.
+// and rho(k); in mixed precision rr and ri hold pre Re rho and pre Im rho.
+// 80 bytes, read with scalar loads.
 struct KRec {
   double kx, ky, kz, pre, mx, my, mz, rr, ri, pad;
 };

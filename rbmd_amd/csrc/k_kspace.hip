@@ -71,6 +71,16 @@ __device__ __forceinline__ void rho_partial_body(const SysView& s, const KView& 
     re[t] = im[t] = 0.0;
   }
   const double ibx = 1.0 / s.box[0], iby = 1.0 / s.box[1], ibz = 1.0 / s.box[2];
+  if constexpr (F32) {
+    // mixed precision: the phase as (m / L).r, so the atom's coordinates are
+    // not scaled per atom (a change of ~1e-15 turns in the phase)
+#pragma unroll
+    for (int t = 0; t < KB; ++t) {
+      mx[t] *= ibx;
+      my[t] *= iby;
+      mz[t] *= ibz;
+    }
+  }
   const int per = kv.rho_atoms / NW;
   const int a0 = ba * kv.rho_atoms + w * per;
   const int a1 = a0 + per < s.n ? a0 + per : s.n;
@@ -83,7 +93,8 @@ __device__ __forceinline__ void rho_partial_body(const SysView& s, const KView& 
   const const_double* Q = (const const_double*)s.q;
 #pragma unroll 2
   for (int i = a0; i < a1; ++i) {
-    const double sx = X[i] * ibx, sy = Y[i] * iby, sz = Z[i] * ibz, q = Q[i];
+    const double sx = F32 ? X[i] : X[i] * ibx, sy = F32 ? Y[i] : Y[i] * iby, sz = F32 ? Z[i] : Z[i] * ibz;
+    const double q = Q[i];
 #pragma unroll
     for (int t = 0; t < KB; ++t) {
       double sn, cs;
@@ -162,28 +173,32 @@ __device__ __forceinline__ void rho_reduce_body(const SysView& s, const KView& k
     for (int b = w; b < kv.nblk_rho; b += kRedWaves) acc += kv.rho_part[(size_t)b * n2 + t];
   part[w][lane] = acc;
   __syncthreads();
-  if (w == 0 && t < n2) {
-    double r = part[0][lane];
-    for (int v = 1; v < kRedWaves; ++v) r += part[v][lane];
-    kv.rho[t] = r;
-    if (kv.krec) {
+  if (w == 0) {
+    double r = 0.0;
+    if (t < n2) {
+      r = part[0][lane];
+      for (int v = 1; v < kRedWaves; ++v) r += part[v][lane];
+      kv.rho[t] = r;
+    }
+    // Im rho from the odd neighbour lane (2P is even: both lanes of a pair
+    // are in range or both out); the whole wavefront takes part
+    const double r_im = __shfl_xor(r, 1);
+    if (kv.krec && t < n2 && !(t & 1)) {
       KRec& rec = kv.krec[t >> 1];
-      if (t & 1) {
-        rec.ri = r;
-      } else {
-        const int l = t >> 1;
-        const double kx = kv.kvec[3 * l], ky = kv.kvec[3 * l + 1], kz = kv.kvec[3 * l + 2];
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        rec.kx = kx;
-        rec.ky = ky;
-        rec.kz = kz;
-        rec.pre = kv.ewald ? exp(-k2 / (4 * s.alpha)) / (s.volume * k2) : 1.0 / (s.volume * k2);
-        rec.mx = k_to_m(kx, s.box[0]);
-        rec.my = k_to_m(ky, s.box[1]);
-        rec.mz = k_to_m(kz, s.box[2]);
-        rec.rr = r;
-        rec.pad = 0.0;
-      }
+      const int l = t >> 1;
+      const double kx = kv.kvec[3 * l], ky = kv.kvec[3 * l + 1], kz = kv.kvec[3 * l + 2];
+      const double k2 = kx * kx + ky * ky + kz * kz;
+      const double pre = kv.ewald ? exp(-k2 / (4 * s.alpha)) / (s.volume * k2) : 1.0 / (s.volume * k2);
+      rec.kx = kx;
+      rec.ky = ky;
+      rec.kz = kz;
+      rec.pre = pre;
+      rec.mx = k_to_m(kx, s.box[0]);
+      rec.my = k_to_m(ky, s.box[1]);
+      rec.mz = k_to_m(kz, s.box[2]);
+      rec.rr = kv.f32 ? pre * r : r;
+      rec.ri = kv.f32 ? pre * r_im : r_im;
+      rec.pad = 0.0;
     }
   }
 }
@@ -257,21 +272,53 @@ __device__ __forceinline__ void kforce_body(const SysView& s, const KView& kv, d
   // written during this kernel)
   typedef const __attribute__((address_space(4))) KRec const_krec;
   const const_krec* rec = (const const_krec*)kv.krec;
-  for (int l = l0; l < l1; ++l) {
-    const double kx = rec[l].kx, ky = rec[l].ky, kz = rec[l].kz;
-    double sn, cs;
-    phase_sincos<F32>(rec[l].mx, rec[l].my, rec[l].mz, sx, sy, sz, &sn, &cs);
-    const double fs = (q4 * rec[l].pre) * fma(cs, rec[l].ri, -(sn * rec[l].rr));
-    fx = fma(fs, kx, fx);
-    fy = fma(fs, ky, fy);
-    fz = fma(fs, kz, fz);
+  if constexpr (F32) {
+    // Mixed precision: the term's coefficient pre Im rho cos - pre Re rho sin
+    // from the record's products (FP64, so that the forces still sum to zero
+    // over the atoms: sum_i q_i (a cos_i - b sin_i) = a Re rho - b Im rho
+    // cancels only for a / b = Im rho / Re rho), the sums in FP64 and the
+    // atom's -4 pi q applied to them.  The virial terms
+    // (x k_a + x k_a) fs summed over k are 2 x F_a (and x F_b + y F_a off the
+    // diagonal), formed from the wavefront's sums.  Two records per
+    // iteration: their scalar loads are in flight together.
+#pragma unroll 2
+    for (int l = l0; l < l1; ++l) {
+      const double t = fma(rec[l].mz, sz, fma(rec[l].my, sy, rec[l].mx * sx));
+      float sn, cs;
+      sincos_turns32_hw(t, &sn, &cs);
+      const double g = fma((double)cs, rec[l].ri, -((double)sn * rec[l].rr));
+      fx = fma(g, rec[l].kx, fx);
+      fy = fma(g, rec[l].ky, fy);
+      fz = fma(g, rec[l].kz, fz);
+    }
+    fx = q4 * fx;
+    fy = q4 * fy;
+    fz = q4 * fz;
     if (THERMO || PERATOM) {
-      v[0] += (x * kx + x * kx) * fs;
-      v[1] += (y * ky + y * ky) * fs;
-      v[2] += (z * kz + z * kz) * fs;
-      v[3] += (x * ky + y * kx) * fs;
-      v[4] += (x * kz + z * kx) * fs;
-      v[5] += (y * kz + z * ky) * fs;
+      v[0] = (x + x) * fx;
+      v[1] = (y + y) * fy;
+      v[2] = (z + z) * fz;
+      v[3] = x * fy + y * fx;
+      v[4] = x * fz + z * fx;
+      v[5] = y * fz + z * fy;
+    }
+  } else {
+    for (int l = l0; l < l1; ++l) {
+      const double kx = rec[l].kx, ky = rec[l].ky, kz = rec[l].kz;
+      double sn, cs;
+      phase_sincos<F32>(rec[l].mx, rec[l].my, rec[l].mz, sx, sy, sz, &sn, &cs);
+      const double fs = (q4 * rec[l].pre) * fma(cs, rec[l].ri, -(sn * rec[l].rr));
+      fx = fma(fs, kx, fx);
+      fy = fma(fs, ky, fy);
+      fz = fma(fs, kz, fz);
+      if (THERMO || PERATOM) {
+        v[0] += (x * kx + x * kx) * fs;
+        v[1] += (y * ky + y * ky) * fs;
+        v[2] += (z * kz + z * kz) * fs;
+        v[3] += (x * ky + y * kx) * fs;
+        v[4] += (x * kz + z * kx) * fs;
+        v[5] += (y * kz + z * ky) * fs;
+      }
     }
   }
   const double sc = kv.S_over_P;
