@@ -487,10 +487,13 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   // Candidates within 1e-5 rc^2 of rc^2 - 1e-4, or at or below 1e-3, are
   // decided by the FP64 predicate of the per-candidate path (the reference's
   // rc^2 - d^2 > 1e-4 && d^2 > 1e-4 on the same rounded vector), all others by
-  // the FP32 comparison, which then agrees with it.  The atom itself (same
-  // cell, no image shift) has d^2 = 0 exactly, so the FP64 predicate rejects
-  // it.  A lane without an i-atom sits at 1e30: its d^2 is +inf, and neither
-  // predicate holds.
+  // the FP32 comparison, which then agrees with it.  An FP32 d^2 of exactly
+  // 0 is rejected without a re-test: it means FP32-equal coordinates, which
+  // differ by at most one FP32 ulp in FP64, so their FP64 d^2 is below
+  // 3 (2^-23 * 3 rc)^2, far under 1e-4.  That is the atom itself (same cell,
+  // no image shift), which every centre column meets once per i-atom.  A lane
+  // without an i-atom sits at 1e30: its d^2 is +inf, and neither predicate
+  // holds.
   const float cut2m_f = (float)(cut2 - 0.0001);
   const float band_f = (float)(1e-5 * cut2);
   // accepted in FP32: 1e-3 < d2 < cut2m - band, as one unsigned range test on
@@ -498,6 +501,9 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   const unsigned in_lo = __float_as_uint(1e-3f) + 1u;
   const unsigned in_w = __float_as_uint(cut2m_f - band_f) - in_lo;
   const float out_f = cut2m_f + band_f;  // rejected in FP32: d2 >= out_f
+  // not rejected in FP32: 0 < d2 < out_f, as one unsigned range test (d2 = 0
+  // wraps to the top)
+  const unsigned nr_w = __float_as_uint(out_f) - 1u;
   const float lix = active ? (float)(pi.x - bl[0]) : 1e30f;
   const float liy = active ? (float)(pi.y - bl[1]) : 1e30f;
   const float liz = active ? (float)(pi.z - bl[2]) : 1e30f;
@@ -518,12 +524,13 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
     // flagged as overflowed (status 4) and not used
     if (ok & ((cnt & 3) == 0)) *reinterpret_cast<int4*>(row + (min(cnt, seg_cap) - 4)) = make_int4(b0, b1, b2, b3);
   };
-  // the FP32 predicate of one staged candidate; nr: decide it in FP64
-  auto test = [&](const float4 q, bool& ok, bool& nr) {
+  // the FP32 predicate of one staged candidate: ok accepted, nx not rejected
+  // (accepted, or decided in FP64 if not accepted)
+  auto test = [&](const float4 q, bool& ok, bool& nx) {
     const float rx = q.x - lix, ry = q.y - liy, rz = q.z - liz;
     const float d2 = rx * rx + ry * ry + rz * rz;
     ok = __float_as_uint(d2) - in_lo < in_w;
-    nr = !ok & (d2 < out_f);
+    nx = __float_as_uint(d2) - 1u < nr_w;
   };
   // the per-candidate FP64 predicate on the build's own vector (the staged
   // shifted position; no global load in the test loop, where a load would
@@ -584,24 +591,27 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
       float4 q[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) q[r] = stage[w][u + r];
-      bool ok[4], nr[4];
+      bool ok[4], nx[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) test(q[r], ok[r], nr[r]);
+      for (int r = 0; r < 4; ++r) test(q[r], ok[r], nx[r]);
       // two copies of the appends, so that in the common one the FP32
-      // decisions stay lane masks
-      if (__ballot(nr[0] | nr[1] | nr[2] | nr[3]) == 0ull) {
+      // decisions stay lane masks; a lane needs the FP64 test where nx and ok
+      // differ (ok implies nx), found on the two masks without a per-lane not
+      const unsigned long long retest = (__ballot(nx[0]) ^ __ballot(ok[0])) | (__ballot(nx[1]) ^ __ballot(ok[1])) |
+                                        (__ballot(nx[2]) ^ __ballot(ok[2])) | (__ballot(nx[3]) ^ __ballot(ok[3]));
+      if (retest == 0ull) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) push(ok[r], __float_as_int(q[r].w));
       } else {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) push(nr[r] ? exact(u + r) : ok[r], __float_as_int(q[r].w));
+        for (int r = 0; r < 4; ++r) push(nx[r] && !ok[r] ? exact(u + r) : ok[r], __float_as_int(q[r].w));
       }
     }
     for (; u < kc; ++u) {
       const float4 q = stage[w][u];
-      bool ok, nr;
-      test(q, ok, nr);
-      if (nr) ok = exact(u);
+      bool ok, nx;
+      test(q, ok, nx);
+      if (nx && !ok) ok = exact(u);
       push(ok, __float_as_int(q.w));
     }
     wave_sync();
