@@ -13,6 +13,8 @@ constexpr int kNbrWaves = 3;        // columns (wavefronts) per list-build workg
 constexpr int kNbrParts = kColWaves / kNbrWaves;  // list-build workgroups per work item
 constexpr int kStageRounds = 2;    // 64-candidate rounds staged per list-build batch
 constexpr int kStageCap = kStageRounds * 64;  // staged candidates per wavefront (one batch)
+constexpr int kListRing = 8;        // list-build LDS ring of accepted entries per lane (two groups of four)
+constexpr int kListRingStride = 9;  // words between two lanes' rings (odd: distinct banks)
 
 // InWhichCell (ExecPointLocator.h:271-274): unclamped trunc
 __device__ __forceinline__ int cell_axis(double x, double lo, double d) {

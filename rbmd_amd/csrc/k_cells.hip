@@ -423,7 +423,8 @@ __device__ void misfit_block(const SysView& s, int wave, int lane, int slot) {
 static_assert(kNbrParts == 3, "the pass order below names three parts");
 __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const int bid) {
   __shared__ float4 stage[kNbrWaves][kStageCap];  // FP32 position from the chunk's box corner, entry bits in w
-  __shared__ double4 stage64[kNbrWaves][kStageCap];  // the same candidates' shifted FP64 positions (re-tests only)
+  __shared__ double stage64[kNbrWaves][kStageCap][3];  // the same candidates' shifted FP64 positions (re-tests only)
+  __shared__ int ring[kNbrWaves][kWave * kListRingStride];  // each lane's last accepted entries
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x & (kWave - 1);
   const int per_pass = s.n_items + kMisfitBlocks;
@@ -507,22 +508,32 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   const float lix = active ? (float)(pi.x - bl[0]) : 1e30f;
   const float liy = active ? (float)(pi.y - bl[1]) : 1e30f;
   const float liz = active ? (float)(pi.z - bl[2]) : 1e30f;
-  // Accepted entries shift through four registers (the newest in b3) and
-  // leave as one 16-byte store per four entries (a segment starts on a
+  // Accepted entries go through a ring of kListRing slots per lane in LDS
+  // and leave as one 16-byte store per four entries (a segment starts on a
   // 64-byte boundary and holds a multiple of 16 slots), so the row is written
   // in whole 16-byte pieces rather than one scattered 4-byte store per entry.
-  // The shift is four selects on the accept mask, with no branch around it.
+  // Every lane writes every candidate's entry at its count and advances the
+  // count only if it accepted, so a rejected entry is overwritten by the next
+  // one: no select and no branch per candidate.  The lanes' rings are
+  // kListRingStride words apart (odd), so a write of the whole wavefront at
+  // equal counts hits distinct banks.
+  int* const my = &ring[w][lane * kListRingStride];
   int cnt = 0;
-  int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
   auto push = [&](bool ok, int e) {
-    b0 = ok ? b1 : b0;
-    b1 = ok ? b2 : b1;
-    b2 = ok ? b3 : b2;
-    b3 = ok ? e : b3;
+    my[cnt & (kListRing - 1)] = e;
     cnt += ok ? 1 : 0;
-    // past the capacity the last four slots are overwritten: the list is then
-    // flagged as overflowed (status 4) and not used
-    if (ok & ((cnt & 3) == 0)) *reinterpret_cast<int4*>(row + (min(cnt, seg_cap) - 4)) = make_int4(b0, b1, b2, b3);
+  };
+  // after at most four pushes since the count was cnt0 the count has passed
+  // at most one multiple of four: the four entries before it are complete in
+  // one half of the ring and go to the row.  Past the capacity the last four
+  // slots are overwritten: the list is then flagged as overflowed (status 4)
+  // and not used.
+  auto flush = [&](int cnt0) {
+    if ((cnt ^ cnt0) & ~3) {
+      const int end = cnt & ~3;
+      const int h = (end - 4) & (kListRing - 1);
+      *reinterpret_cast<int4*>(row + (min(end, seg_cap) - 4)) = make_int4(my[h], my[h + 1], my[h + 2], my[h + 3]);
+    }
   };
   // the FP32 predicate of one staged candidate: ok accepted, nx not rejected
   // (accepted, or decided in FP64 if not accepted)
@@ -536,10 +547,9 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
   // shifted position; no global load in the test loop, where a load would
   // make every iteration wait for the row stores before it)
   auto exact = [&](int slot) -> bool {
-    const double4 q = stage64[w][slot];
-    const double rx = q.x - pi.x;
-    const double ry = q.y - pi.y;
-    const double rz = q.z - pi.z;
+    const double rx = stage64[w][slot][0] - pi.x;
+    const double ry = stage64[w][slot][1] - pi.y;
+    const double rz = stage64[w][slot][2] - pi.z;
     const double d2 = dist2(rx, ry, rz);
     return (cut2 - d2 > 0.0001) & (d2 > 0.0001);
   };
@@ -577,7 +587,9 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
         const int slot = kc + __popcll(bal & ((1ull << lane) - 1ull));
         stage[w][slot] = make_float4((float)(pj.x - bl[0]), (float)(pj.y - bl[1]), (float)(pj.z - bl[2]),
                                      __int_as_float(er[r]));
-        stage64[w][slot] = pj;
+        stage64[w][slot][0] = pj.x;
+        stage64[w][slot][1] = pj.y;
+        stage64[w][slot][2] = pj.z;
       }
       kc += __popcll(bal);
     }
@@ -599,6 +611,7 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
       // differ (ok implies nx), found on the two masks without a per-lane not
       const unsigned long long retest = (__ballot(nx[0]) ^ __ballot(ok[0])) | (__ballot(nx[1]) ^ __ballot(ok[1])) |
                                         (__ballot(nx[2]) ^ __ballot(ok[2])) | (__ballot(nx[3]) ^ __ballot(ok[3]));
+      const int cnt0 = cnt;
       if (retest == 0ull) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) push(ok[r], __float_as_int(q[r].w));
@@ -606,24 +619,26 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
 #pragma unroll
         for (int r = 0; r < 4; ++r) push(nx[r] && !ok[r] ? exact(u + r) : ok[r], __float_as_int(q[r].w));
       }
+      flush(cnt0);
     }
     for (; u < kc; ++u) {
       const float4 q = stage[w][u];
       bool ok, nx;
       test(q, ok, nx);
       if (nx && !ok) ok = exact(u);
+      const int cnt0 = cnt;
       push(ok, __float_as_int(q.w));
+      flush(cnt0);
     }
     wave_sync();
   }
-  {  // the last (cnt & 3) entries: b(4 - k) ... b3
+  {  // the last (cnt & 3) entries, in the ring from slot first & (kListRing - 1)
     const int k = cnt & 3;
     const int first = cnt - k;
-    const int e0 = k == 3 ? b1 : (k == 2 ? b2 : b3);
-    const int e1 = k == 3 ? b2 : b3;
-    if (k > 0 && first < seg_cap) row[first] = e0;
-    if (k > 1 && first + 1 < seg_cap) row[first + 1] = e1;
-    if (k > 2 && first + 2 < seg_cap) row[first + 2] = b3;
+    const int h = first & (kListRing - 1);
+    if (k > 0 && first < seg_cap) row[first] = my[h];
+    if (k > 1 && first + 1 < seg_cap) row[first + 1] = my[h + 1];
+    if (k > 2 && first + 2 < seg_cap) row[first + 2] = my[h + 2];
   }
   if (cnt > seg_cap) atomicOr(s.status, 4);
   if (active) s.nbr_num[wave * s.n + i] = cnt < seg_cap ? cnt : seg_cap;
