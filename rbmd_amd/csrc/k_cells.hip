@@ -597,12 +597,10 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
     wave_sync();
     // every staged candidate is tested by every lane; an accepting lane
     // appends the candidate's entry to its own row in stream order.  Four
-    // candidates are read per LDS round trip.
-    int u = 0;
-    for (; u + 4 <= kc; u += 4) {
-      float4 q[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) q[r] = stage[w][u + r];
+    // candidates are read per LDS round trip, into two sets of registers
+    // used in turn: the next group's reads are issued before this group's
+    // test and appends, so their latency overlaps them.
+    auto group = [&](const float4 (&q)[4], int u) {
       bool ok[4], nx[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) test(q[r], ok[r], nx[r]);
@@ -620,6 +618,23 @@ __device__ __forceinline__ void nbr_build_verlet_body(const SysView& s, const in
         for (int r = 0; r < 4; ++r) push(nx[r] && !ok[r] ? exact(u + r) : ok[r], __float_as_int(q[r].w));
       }
       flush(cnt0);
+    };
+    auto read4 = [&](float4 (&q)[4], int u) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q[r] = stage[w][u + r];
+    };
+    int u = 0;
+    float4 qa[4], qb[4];
+    if (kc >= 4) read4(qa, 0);
+    for (; u + 8 <= kc; u += 8) {
+      read4(qb, u + 4);
+      group(qa, u);
+      if (u + 12 <= kc) read4(qa, u + 8);
+      group(qb, u + 4);
+    }
+    if (u + 4 <= kc) {  // qa holds the group at u
+      group(qa, u);
+      u += 4;
     }
     for (; u < kc; ++u) {
       const float4 q = stage[w][u];
