@@ -34,26 +34,47 @@ __device__ __forceinline__ int pick_seg(const int (&c)[kMaxSeg], int k) {
   return r;
 }
 
+// Raw buffer loads (32-bit byte offsets from a wave-uniform base, bounds
+// checked by the hardware: a load past num_records returns zeros).  The pair
+// loop's gathers and list reads take their addresses as one 32-bit shift or
+// shift-add instead of 64-bit address arithmetic.
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+typedef unsigned BufV4 __attribute__((ext_vector_type(4)));
+typedef double BufD2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ BufRsrc buf_rsrc(const void* p, unsigned bytes) {
+  // word 3 of a gfx9 buffer resource: 32-bit data format, untyped loads
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double4 buf_double4(BufRsrc r, unsigned off) {
+  const BufD2 a = __builtin_bit_cast(BufD2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  const BufD2 b = __builtin_bit_cast(BufD2, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0));
+  return make_double4(a.x, a.y, b.x, b.y);
+}
+__device__ __forceinline__ int buf_int(BufRsrc r, unsigned off) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+
 // A lane's position in an atom's concatenated segments: `seg` (nseg past the
-// end), the offset in it, its length (INT_MAX past the end) and its start.
+// end), the offset in it, its length (INT_MAX past the end) and the byte
+// offset of the segment's start from the wavefront's first list row.
 struct ListCursor {
   int seg, off, num;
-  const int* p;
+  unsigned po;
 };
 
 // entry at the cursor (-1 past the end), then one stride forward; crossing
 // into the next segment is the rare path
-__device__ __forceinline__ int cursor_take(ListCursor& c, const int (&cnt)[kMaxSeg], int nseg, int seg_cap,
-                                           int stride) {
+__device__ __forceinline__ int cursor_take(ListCursor& c, BufRsrc rows, const int (&cnt)[kMaxSeg], int nseg,
+                                           int seg_cap, int stride) {
   if (__builtin_expect(c.off >= c.num, 0)) {
     do {
       c.off -= c.num;
       ++c.seg;
-      c.p += seg_cap;
+      c.po += 4u * (unsigned)seg_cap;
       c.num = c.seg < nseg ? pick_seg(cnt, c.seg) : 0x7fffffff;
     } while (c.off >= c.num);
   }
-  const int e = c.seg < nseg ? c.p[c.off] : -1;
+  const int e = c.seg < nseg ? buf_int(rows, c.po + 4u * (unsigned)c.off) : -1;
   c.off += stride;
   return e;
 }
@@ -123,25 +144,34 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
 #pragma unroll
     for (int g = 0; g < kMaxSeg; ++g) cnt[g] = g < s.nseg ? s.nbr_num[g * s.n + i] : 0;
     const int nseg = s.nseg, seg_cap = s.seg_cap;
-    ListCursor cur{0, sub, cnt[0], s.nbr + (size_t)i * s.cap};
+    // the wavefront's list rows (its 64 / L atoms, fewer in the last one)
+    // and the sorted positions and types as buffers
+    const int i0 = __builtin_amdgcn_readfirstlane(i);
+    const int rows_here = min(kWave >> lg, s.n - i0);
+    const BufRsrc rrows = buf_rsrc(s.nbr + (size_t)i0 * s.cap, 4u * (unsigned)s.cap * (unsigned)rows_here);
+    const BufRsrc rpos = buf_rsrc(s.spos, 32u * (unsigned)s.n);
+    const BufRsrc rtyp = buf_rsrc(s.stype, 4u * (unsigned)s.n);
+    ListCursor cur{0, sub, cnt[0], 4u * (unsigned)(i - i0) * (unsigned)s.cap};
     // pipeline: entries two iterations ahead, positions one iteration ahead
-    int ea = cursor_take(cur, cnt, nseg, seg_cap, L);
-    int eb = cursor_take(cur, cnt, nseg, seg_cap, L);
-    // neighbour index of an entry (the atom itself for -1: a harmless load)
-    auto jof = [i](int e) -> unsigned { return e >= 0 ? (unsigned)e >> kShiftBits : (unsigned)i; };
-    double4 pa = s.spos[jof(ea)];
-    double4 pb = s.spos[jof(eb)];
-    int ta = s.stype[jof(ea)];
-    int tb = s.stype[jof(eb)];
-    int na = cursor_take(cur, cnt, nseg, seg_cap, L);
-    int nb = cursor_take(cur, cnt, nseg, seg_cap, L);
+    int ea = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
+    int eb = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
+    // an entry's neighbour as a byte offset into the positions; -1 (past the
+    // end) gives 2^23 - 1, beyond every buffer (n < 2^22): the loads return
+    // zeros, which the loop never uses
+    auto jof = [](int e) -> unsigned { return (unsigned)e >> kShiftBits; };
+    double4 pa = buf_double4(rpos, 32u * jof(ea));
+    double4 pb = buf_double4(rpos, 32u * jof(eb));
+    int ta = buf_int(rtyp, 4u * jof(ea));
+    int tb = buf_int(rtyp, 4u * jof(eb));
+    int na = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
+    int nb = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
     while (ea >= 0) {
-      const double4 qa = s.spos[jof(na)];
-      const double4 qb = s.spos[jof(nb)];
-      const int ua = s.stype[jof(na)];
-      const int ub = s.stype[jof(nb)];
-      const int ma = cursor_take(cur, cnt, nseg, seg_cap, L);
-      const int mb = cursor_take(cur, cnt, nseg, seg_cap, L);
+      const double4 qa = buf_double4(rpos, 32u * jof(na));
+      const double4 qb = buf_double4(rpos, 32u * jof(nb));
+      const int ua = buf_int(rtyp, 4u * jof(na));
+      const int ub = buf_int(rtyp, 4u * jof(nb));
+      const int ma = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
+      const int mb = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int e = h == 0 ? ea : eb;
