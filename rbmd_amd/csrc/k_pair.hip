@@ -27,13 +27,6 @@ namespace rbmd {
 
 constexpr int kMaxSeg = 9;  // VERLET list segments per atom (stencil columns)
 
-__device__ __forceinline__ int pick_seg(const int (&c)[kMaxSeg], int k) {
-  int r = c[0];
-#pragma unroll
-  for (int t = 1; t < kMaxSeg; ++t) r = k == t ? c[t] : r;
-  return r;
-}
-
 // Raw buffer loads (32-bit byte offsets from a wave-uniform base, bounds
 // checked by the hardware: a load past num_records returns zeros).  The pair
 // loop's gathers and list reads take their addresses as one 32-bit shift or
@@ -63,15 +56,17 @@ struct ListCursor {
 };
 
 // entry at the cursor (-1 past the end), then one stride forward; crossing
-// into the next segment is the rare path
-__device__ __forceinline__ int cursor_take(ListCursor& c, BufRsrc rows, const int (&cnt)[kMaxSeg], int nseg,
-                                           int seg_cap, int stride) {
+// into the next segment is the rare path.  seg_cnt: the workgroup's segment
+// counts in LDS, one column per thread, so a crossing reads the next count at
+// an address computed from the segment index.
+__device__ __forceinline__ int cursor_take(ListCursor& c, BufRsrc rows, const int (&seg_cnt)[kMaxSeg][kBlock],
+                                           int nseg, int seg_cap, int stride) {
   if (__builtin_expect(c.off >= c.num, 0)) {
     do {
       c.off -= c.num;
       ++c.seg;
       c.po += 4u * (unsigned)seg_cap;
-      c.num = c.seg < nseg ? pick_seg(cnt, c.seg) : 0x7fffffff;
+      c.num = c.seg < nseg ? seg_cnt[c.seg][threadIdx.x] : 0x7fffffff;
     } while (c.off >= c.num);
   }
   const int e = c.seg < nseg ? buf_int(rows, c.po + 4u * (unsigned)c.off) : -1;
@@ -98,6 +93,7 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
   constexpr int kTab = F32 ? kCoulTab32 : kCoulTab;
   __shared__ TT teps[LTAB ? kMaxTypes * kMaxTypes : 1], tsig[LTAB ? kMaxTypes * kMaxTypes : 1];
   __shared__ CT ctab[MODE == 2 ? 1 : kTab];
+  __shared__ int seg_cnt[kMaxSeg][kBlock];  // each thread's list segment counts (one column per thread)
   const int nt = s.ntypes;
   if (LTAB)
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) {
@@ -140,9 +136,9 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
         spw[k] = k < spn ? s.sp_w[b + k] : 1.0;
       }
     }
-    int cnt[kMaxSeg];
 #pragma unroll
-    for (int g = 0; g < kMaxSeg; ++g) cnt[g] = g < s.nseg ? s.nbr_num[g * s.n + i] : 0;
+    for (int g = 0; g < kMaxSeg; ++g) seg_cnt[g][threadIdx.x] = g < s.nseg ? s.nbr_num[g * s.n + i] : 0;
+    const int cnt0 = s.nseg > 0 ? s.nbr_num[i] : 0;
     const int nseg = s.nseg, seg_cap = s.seg_cap;
     // the wavefront's list rows (its 64 / L atoms, fewer in the last one)
     // and the sorted positions and types as buffers
@@ -151,10 +147,10 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
     const BufRsrc rrows = buf_rsrc(s.nbr + (size_t)i0 * s.cap, 4u * (unsigned)s.cap * (unsigned)rows_here);
     const BufRsrc rpos = buf_rsrc(s.spos, 32u * (unsigned)s.n);
     const BufRsrc rtyp = buf_rsrc(s.stype, 4u * (unsigned)s.n);
-    ListCursor cur{0, sub, cnt[0], 4u * (unsigned)(i - i0) * (unsigned)s.cap};
+    ListCursor cur{0, sub, cnt0, 4u * (unsigned)(i - i0) * (unsigned)s.cap};
     // pipeline: entries two iterations ahead, positions one iteration ahead
-    int ea = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
-    int eb = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
+    int ea = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
+    int eb = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
     // an entry's neighbour as a byte offset into the positions; -1 (past the
     // end) gives 2^23 - 1, beyond every buffer (n < 2^22): the loads return
     // zeros, which the loop never uses
@@ -163,15 +159,15 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
     double4 pb = buf_double4(rpos, 32u * jof(eb));
     int ta = buf_int(rtyp, 4u * jof(ea));
     int tb = buf_int(rtyp, 4u * jof(eb));
-    int na = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
-    int nb = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
+    int na = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
+    int nb = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
     while (ea >= 0) {
       const double4 qa = buf_double4(rpos, 32u * jof(na));
       const double4 qb = buf_double4(rpos, 32u * jof(nb));
       const int ua = buf_int(rtyp, 4u * jof(na));
       const int ub = buf_int(rtyp, 4u * jof(nb));
-      const int ma = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
-      const int mb = cursor_take(cur, rrows, cnt, nseg, seg_cap, L);
+      const int ma = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
+      const int mb = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int e = h == 0 ? ea : eb;
