@@ -55,7 +55,8 @@ struct ListCursor {
   unsigned po;
 };
 
-// entry at the cursor (-1 past the end), then one stride forward; crossing
+// entry at the cursor (0 past the end: a list entry is never 0, its image
+// code is at least 2 + (2 << 3) + (2 << 6)), then one stride forward; crossing
 // into the next segment is the rare path.  seg_cnt: the workgroup's segment
 // counts in LDS, one column per thread, so a crossing reads the next count at
 // an address computed from the segment index.
@@ -65,11 +66,13 @@ __device__ __forceinline__ int cursor_take(ListCursor& c, BufRsrc rows, const in
     do {
       c.off -= c.num;
       ++c.seg;
-      c.po += 4u * (unsigned)seg_cap;
+      // past the last segment the offset lies beyond the buffer (2^31 and
+      // more), where loads return 0
+      c.po = c.seg < nseg ? c.po + 4u * (unsigned)seg_cap : 0x80000000u;
       c.num = c.seg < nseg ? seg_cnt[c.seg][threadIdx.x] : 0x7fffffff;
     } while (c.off >= c.num);
   }
-  const int e = c.seg < nseg ? buf_int(rows, c.po + 4u * (unsigned)c.off) : -1;
+  const int e = buf_int(rows, c.po + 4u * (unsigned)c.off);
   c.off += stride;
   return e;
 }
@@ -151,9 +154,8 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
     // pipeline: entries two iterations ahead, positions one iteration ahead
     int ea = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
     int eb = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
-    // an entry's neighbour as a byte offset into the positions; -1 (past the
-    // end) gives 2^23 - 1, beyond every buffer (n < 2^22): the loads return
-    // zeros, which the loop never uses
+    // an entry's neighbour as a byte offset into the positions (0 past the
+    // end: atom 0, loaded and not used)
     auto jof = [](int e) -> unsigned { return (unsigned)e >> kShiftBits; };
     double4 pa = buf_double4(rpos, 32u * jof(ea));
     double4 pb = buf_double4(rpos, 32u * jof(eb));
@@ -161,7 +163,7 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
     int tb = buf_int(rtyp, 4u * jof(eb));
     int na = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
     int nb = cursor_take(cur, rrows, seg_cnt, nseg, seg_cap, L);
-    while (ea >= 0) {
+    while (ea > 0) {
       const double4 qa = buf_double4(rpos, 32u * jof(na));
       const double4 qb = buf_double4(rpos, 32u * jof(nb));
       const int ua = buf_int(rtyp, 4u * jof(na));
@@ -171,7 +173,7 @@ __device__ __forceinline__ void pair_verlet_body(const SysView& s, double* rows,
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int e = h == 0 ? ea : eb;
-        if (e < 0) continue;
+        if (e <= 0) continue;
         const double4 p = h == 0 ? pa : pb;
         const int tj = h == 0 ? ta : tb;
         int j;
